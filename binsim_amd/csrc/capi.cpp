@@ -1,0 +1,583 @@
+// extern "C" boundary of libhipotf.  Declarations and reference citations: include/hipotf.h.
+//
+// Error convention: no entry point exits or throws across the boundary.  Failures record a
+// thread-local message (hipotf_last_error) and return NULL / NaN / nonzero status; void entry
+// points only record the message.
+#include "hipotf.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <map>
+#include <mutex>
+#include <unordered_map>
+
+namespace hipotf {
+const char *last_error_cstr();
+double ritz_and_residual(i64 N, int dim, const double *coeffs, double min_e, const double *const *V,
+                         const double *const *AV, double *axt, hipStream_t st);
+void ritz_vector(i64 N, int dim, const double *coeffs, const double *const *V, double *xt, hipStream_t st);
+void precondition(i64 N, double *axt, const double *diags, double min_e, double shift, hipStream_t st);
+double mgs(i64 N, int dim, const double *const *V, double *vt, hipStream_t st);
+double normalize(i64 N, double *vec, hipStream_t st);
+double dot(i64 N, const double *a, const double *b, hipStream_t st);
+void dot_dev(i64 N, const double *a, const double *b, double *dres, hipStream_t st);
+void multidot(i64 N, int dim, const double *const *V, const double *a, double *out_host, hipStream_t st);
+
+// ------------------------------------------------------------------------------------------
+// host network: the flat SVD-group arrays of build_network_otf (src/include/ham/otf.hpp:223-349)
+// ------------------------------------------------------------------------------------------
+struct HostGroup {
+    u32 ax, bx;
+    int rank, nza, nzb;
+    std::vector<u32> zas, zbs;
+    std::vector<double> wa, wb;  // [r * nz + k]
+};
+
+struct HostNet {
+    int norb = 0;
+    std::vector<HostGroup> groups;
+    std::unordered_map<u64, int> by_x;  // (ax << 32 | bx) -> group
+    // f_g(a, b) = sum_r Pa_r(a) Pb_r(b), P_r(s) = sum_k w[r,k] (-1)^popc(s & z_k)
+    // (src/include/core/math.hpp:37-124); the matrix element <(a,b)^x| H |(a,b)>.
+    double element(u32 ax, u32 bx, u32 a, u32 b) const {
+        auto it = by_x.find(((u64)ax << 32) | bx);
+        if (it == by_x.end()) return 0.0;
+        const HostGroup &g = groups[it->second];
+        double v = 0.0;
+        for (int r = 0; r < g.rank; ++r) {
+            double pa = 0.0, pb = 0.0;
+            for (int k = 0; k < g.nza; ++k)
+                pa += (__builtin_popcount(a & g.zas[k]) & 1) ? -g.wa[r * g.nza + k] : g.wa[r * g.nza + k];
+            for (int k = 0; k < g.nzb; ++k)
+                pb += (__builtin_popcount(b & g.zbs[k]) & 1) ? -g.wb[r * g.nzb + k] : g.wb[r * g.nzb + k];
+            v += pa * pb;
+        }
+        return v;
+    }
+};
+
+// ------------------------------------------------------------------------------------------
+// device operator handle: either a Hamiltonian (integrals) or a generator pool; the
+// basis-dependent tables are built on first use with a basis and cached under its uid
+// ------------------------------------------------------------------------------------------
+struct DevNet {
+    int kind = 0;  // 1 Hamiltonian, 2 pool
+    int n = 0;
+    double ecore = 0.0;
+    std::vector<double> h, eri;
+    std::vector<int32_t> offsets, orbs;
+    std::map<i64, std::unique_ptr<DevHam>> hams;
+    std::map<i64, std::unique_ptr<DevPool>> pools;
+    DevHam *ham_for(DevBasis *b) {
+        HIPOTF_REQUIRE(kind == 1, "operator handle is not a Hamiltonian");
+        auto it = hams.find(b->uid);
+        if (it != hams.end()) return it->second.get();
+        DevHam *H = make_dev_ham(b, n, h, eri, ecore);
+        hams[b->uid].reset(H);
+        return H;
+    }
+    DevPool *pool_for(DevBasis *b) {
+        HIPOTF_REQUIRE(kind == 2, "operator handle is not a generator pool");
+        auto it = pools.find(b->uid);
+        if (it != pools.end()) return it->second.get();
+        HIPOTF_REQUIRE(n == b->host.norb, "orbital count of the pool does not match the basis");
+        DevPool *P = make_dev_pool(b, (i64)offsets.size() - 1, offsets.data(), orbs.data());
+        pools[b->uid].reset(P);
+        return P;
+    }
+};
+
+static inline hipStream_t S0() { return (hipStream_t)0; }
+
+static DevBasis *as_basis(void *p) {
+    HIPOTF_REQUIRE(p != nullptr, "device basis handle is NULL");
+    return static_cast<DevBasis *>(p);
+}
+static DevNet *as_net(void *p) {
+    HIPOTF_REQUIRE(p != nullptr, "device operator handle is NULL");
+    return static_cast<DevNet *>(p);
+}
+
+// ---- decoding a reference-style network by probing matrix elements -----------------------
+static double probe_diag(const HostNet &N, u32 a, u32 b) { return N.element(0, 0, a, b); }
+
+static void decode_hamiltonian(const HostNet &N, DevNet &D) {
+    const int n = N.norb;
+    D.kind = 1;
+    D.n = n;
+    D.h.assign((size_t)n * n, 0.0);
+    D.eri.assign((size_t)n * n * n * n, 0.0);
+    auto E = [&](int p, int q, int r, int s) -> double & {
+        return D.eri[(((size_t)p * n + q) * n + r) * n + s];
+    };
+    const double e0 = probe_diag(N, 0, 0);
+    D.ecore = e0;
+    std::vector<double> ea(n), eb(n);
+    for (int p = 0; p < n; ++p) {
+        ea[p] = probe_diag(N, 1u << p, 0);
+        eb[p] = probe_diag(N, 0, 1u << p);
+        D.h[p * n + p] = ea[p] - e0;
+    }
+    for (int p = 0; p < n; ++p)
+        for (int q = 0; q < n; ++q)
+            if (p != q) D.h[p * n + q] = N.element((1u << p) | (1u << q), 0, 1u << q, 0);
+    // Coulomb (pp|rr) from two-electron diagonals
+    for (int p = 0; p < n; ++p)
+        for (int r = 0; r < n; ++r) E(p, p, r, r) = probe_diag(N, 1u << p, 1u << r) - ea[p] - eb[r] + e0;
+    // (pq|rr), p != q: alpha move q -> p with a spectator beta electron in r; the interleaved
+    // sign is (-1)^[min(p,q) <= r < max(p,q)]
+    for (int p = 0; p < n; ++p)
+        for (int q = 0; q < n; ++q) {
+            if (p == q) continue;
+            for (int r = 0; r < n; ++r) {
+                double f = N.element((1u << p) | (1u << q), 0, 1u << q, 1u << r);
+                const int lo = std::min(p, q), hi = std::max(p, q);
+                if (r >= lo && r < hi) f = -f;
+                E(p, q, r, r) = f - D.h[p * n + q];
+            }
+        }
+    // (pq|rs), p != q, r != s: alpha q -> p and beta s -> r on a two-electron state; gauge signs
+    // (-1)^([r < p] + [s < q]) convert interleaved to alpha-major order
+    for (int p = 0; p < n; ++p)
+        for (int q = 0; q < n; ++q) {
+            if (p == q) continue;
+            for (int r = 0; r < n; ++r)
+                for (int s = 0; s < n; ++s) {
+                    if (r == s) continue;
+                    double f = N.element((1u << p) | (1u << q), (1u << r) | (1u << s), 1u << q, 1u << s);
+                    if (((r < p) + (s < q)) & 1) f = -f;
+                    E(p, q, r, s) = f;
+                }
+        }
+    // (pp|rs), r != s by symmetry
+    for (int p = 0; p < n; ++p)
+        for (int r = 0; r < n; ++r)
+            for (int s = 0; s < n; ++s)
+                if (r != s) E(p, p, r, s) = E(r, s, p, p);
+}
+
+// recognise each pool group as +-(T - T^dagger) for a product T of ladder operators
+static void decode_pool(const HostNet &N, DevNet &D) {
+    const int n = N.norb;
+    D.kind = 2;
+    D.n = n;
+    D.offsets.assign(1, 0);
+    D.orbs.clear();
+    for (size_t g = 0; g < N.groups.size(); ++g) {
+        const HostGroup &G = N.groups[g];
+        const u32 ax = G.ax, bx = G.bx;
+        const int ka = __builtin_popcount(ax), kb = __builtin_popcount(bx);
+        HIPOTF_REQUIRE(ka % 2 == 0 && kb % 2 == 0 && ka + kb >= 2 && ka + kb <= 8,
+                       "pool group " + std::to_string(g) + " is not a number-conserving excitation");
+        std::vector<int> oa, ob;
+        for (int p = 0; p < n; ++p) {
+            if ((ax >> p) & 1) oa.push_back(p);
+            if ((bx >> p) & 1) ob.push_back(p);
+        }
+        // try every split of the alpha (beta) orbitals into annihilated / created halves
+        bool found = false;
+        const int na = ka / 2, nb = kb / 2;
+        for (u32 ma = 0; ma < (1u << ka) && !found; ++ma) {
+            if (__builtin_popcount(ma) != na) continue;
+            for (u32 mb = 0; mb < (1u << kb) && !found; ++mb) {
+                if (__builtin_popcount(mb) != nb) continue;
+                u32 anna = 0, annb = 0;
+                for (int i = 0; i < ka; ++i)
+                    if ((ma >> i) & 1) anna |= 1u << oa[i];
+                for (int i = 0; i < kb; ++i)
+                    if ((mb >> i) & 1) annb |= 1u << ob[i];
+                const double f = N.element(ax, bx, anna, annb);
+                if (std::fabs(std::fabs(f) - 1.0) > 1e-9) continue;
+                // T = a+_{cre a..} a+_{cre b..} a_{ann a..} a_{ann b..} (jl/ansatz.jl order)
+                std::vector<int> term;
+                for (int p = 0; p < n; ++p)
+                    if (((ax & ~anna) >> p) & 1) term.push_back(2 * p);
+                for (int p = 0; p < n; ++p)
+                    if (((bx & ~annb) >> p) & 1) term.push_back(2 * p + 1);
+                for (int p = 0; p < n; ++p)
+                    if ((anna >> p) & 1) term.push_back(2 * p);
+                for (int p = 0; p < n; ++p)
+                    if ((annb >> p) & 1) term.push_back(2 * p + 1);
+                // sign of T on the probe state: apply ladder ops right to left
+                u32 a = anna, b = annb;
+                int par = 0;
+                for (int i = (int)term.size() - 1; i >= 0; --i) {
+                    const int pos = term[i], q = pos >> 1;
+                    if (pos & 1) {
+                        par += __builtin_popcount(a & ((2u << q) - 1)) + __builtin_popcount(b & ((1u << q) - 1));
+                        b ^= 1u << q;
+                    } else {
+                        par += __builtin_popcount(a & ((1u << q) - 1)) + __builtin_popcount(b & ((1u << q) - 1));
+                        a ^= 1u << q;
+                    }
+                }
+                const double t = (par & 1) ? -1.0 : 1.0;
+                if (f * t < 0) {
+                    // group equals -(T - T^dagger) = (T' - T'^dagger) with T' = T^dagger
+                    std::vector<int> rev;
+                    const int h = (int)term.size() / 2;
+                    for (int i = h; i < (int)term.size(); ++i) rev.push_back(term[i]);
+                    for (int i = 0; i < h; ++i) rev.push_back(term[i]);
+                    // T^dagger = a+_{ann..}(reversed) a_{cre..}(reversed): reverse each half
+                    std::reverse(rev.begin(), rev.begin() + h);
+                    std::reverse(rev.begin() + h, rev.end());
+                    term = rev;
+                }
+                for (int v : term) D.orbs.push_back(v);
+                D.offsets.push_back((int32_t)D.orbs.size());
+                found = true;
+            }
+        }
+        HIPOTF_REQUIRE(found, "pool group " + std::to_string(g) +
+                                  " is not a fermionic excitation generator (generic networks unsupported)");
+    }
+}
+
+}  // namespace hipotf
+
+using namespace hipotf;
+
+#define API_TRY try {
+#define API_CATCH(ret)                                        \
+    }                                                         \
+    catch (const std::exception &e) {                         \
+        set_last_error(e.what());                             \
+        return ret;                                           \
+    }                                                         \
+    catch (...) {                                             \
+        set_last_error("unknown error");                      \
+        return ret;                                           \
+    }
+#define API_CATCH_VOID                                        \
+    }                                                         \
+    catch (const std::exception &e) {                         \
+        set_last_error(e.what());                             \
+    }                                                         \
+    catch (...) {                                             \
+        set_last_error("unknown error");                      \
+    }
+
+static const double kNaN = std::numeric_limits<double>::quiet_NaN();
+
+extern "C" {
+
+const char *hipotf_last_error(void) { return last_error_cstr(); }
+void hipotf_clear_error(void) { clear_last_error(); }
+
+// ---- host basis (libbasis) -----------------------------------------------------------------
+void *create_basis_manager(int64_t norb, int64_t na, int64_t nb, int64_t total_sym,
+                           const int64_t *orbsym, int64_t num_irreps) {
+    API_TRY
+    return make_host_basis(norb, na, nb, total_sym, orbsym, num_irreps);
+    API_CATCH(nullptr)
+}
+
+int64_t get_subspace_dim(void *basis) {
+    API_TRY
+    HIPOTF_REQUIRE(basis, "basis handle is NULL");
+    return static_cast<HostBasis *>(basis)->dim;
+    API_CATCH(-1)
+}
+
+int64_t get_num_symmetry_blocks(void *basis) {
+    API_TRY
+    HIPOTF_REQUIRE(basis, "basis handle is NULL");
+    return static_cast<HostBasis *>(basis)->num_blocks;
+    API_CATCH(-1)
+}
+
+void destroy_basis_manager(void *basis) { delete static_cast<HostBasis *>(basis); }
+
+void set_det_coeff_f64(void *basis, uint32_t astr, uint32_t bstr, double coeff, double *vec) {
+    API_TRY
+    HIPOTF_REQUIRE(basis && vec, "NULL argument");
+    const HostBasis *hb = static_cast<HostBasis *>(basis);
+    if (hb->dim == 0) return;
+    auto ia = std::lower_bound(hb->A.strs.begin(), hb->A.strs.end(), (u64)astr);
+    auto ib = std::lower_bound(hb->B.strs.begin(), hb->B.strs.end(), (u64)bstr);
+    if (ia == hb->A.strs.end() || *ia != astr || ib == hb->B.strs.end() || *ib != bstr) return;
+    vec[(ia - hb->A.strs.begin()) * hb->B.N + (ib - hb->B.strs.begin())] += coeff;
+    API_CATCH_VOID
+}
+
+// ---- host network (libotf_native build/destroy) ----------------------------------------------
+void *build_network_otf_f64(const int64_t *orbsym, int64_t norb, int64_t ngs, const uint32_t *axs,
+                            const uint32_t *bxs, const int64_t *ranks, const int64_t *num_zas,
+                            const int64_t *num_zbs, const uint32_t *flat_zas, const uint32_t *flat_zbs,
+                            const double *flat_wa, const double *flat_wb) {
+    API_TRY
+    (void)orbsym;
+    HIPOTF_REQUIRE(norb >= 1 && norb <= 32 && ngs >= 0, "bad network sizes");
+    auto N = std::make_unique<HostNet>();
+    N->norb = (int)norb;
+    size_t za = 0, zb = 0, wa = 0, wb = 0;
+    for (int64_t g = 0; g < ngs; ++g) {
+        HostGroup G;
+        G.ax = axs[g];
+        G.bx = bxs[g];
+        G.rank = (int)ranks[g];
+        G.nza = (int)num_zas[g];
+        G.nzb = (int)num_zbs[g];
+        HIPOTF_REQUIRE(G.rank >= 0 && G.nza >= 0 && G.nzb >= 0, "bad group sizes");
+        G.zas.assign(flat_zas + za, flat_zas + za + G.nza);
+        G.zbs.assign(flat_zbs + zb, flat_zbs + zb + G.nzb);
+        G.wa.assign(flat_wa + wa, flat_wa + wa + (size_t)G.nza * G.rank);
+        G.wb.assign(flat_wb + wb, flat_wb + wb + (size_t)G.nzb * G.rank);
+        za += G.nza;
+        zb += G.nzb;
+        wa += (size_t)G.nza * G.rank;
+        wb += (size_t)G.nzb * G.rank;
+        N->by_x[((u64)G.ax << 32) | G.bx] = (int)N->groups.size();
+        N->groups.push_back(std::move(G));
+    }
+    return N.release();
+    API_CATCH(nullptr)
+}
+
+void destroy_network_otf_f64(void *net) { delete static_cast<HostNet *>(net); }
+
+// ---- device handles (libcuotf) ---------------------------------------------------------------
+void sync_device_cuda(void) {
+    API_TRY
+    HIPOTF_CHECK(hipDeviceSynchronize());
+    API_CATCH_VOID
+}
+
+void *build_basisdev_f64(void *host_basis) {
+    API_TRY
+    HIPOTF_REQUIRE(host_basis, "host basis handle is NULL");
+    return make_dev_basis(*static_cast<HostBasis *>(host_basis));
+    API_CATCH(nullptr)
+}
+
+void destroy_basisdev_f64(void *basis) { delete static_cast<DevBasis *>(basis); }
+
+void *build_networkdev_f64(void *host_net) {
+    API_TRY
+    HIPOTF_REQUIRE(host_net, "host network handle is NULL");
+    const HostNet &N = *static_cast<HostNet *>(host_net);
+    auto D = std::make_unique<DevNet>();
+    bool has_diag = N.by_x.count(0) > 0;
+    if (has_diag) decode_hamiltonian(N, *D);
+    else decode_pool(N, *D);
+    return D.release();
+    API_CATCH(nullptr)
+}
+
+void destroy_networkdev_f64(void *net) { delete static_cast<DevNet *>(net); }
+
+void *hipotf_build_ham_f64(int64_t norb, const double *one_body_mo, const double *two_body_mo,
+                           double energy_nuc) {
+    API_TRY
+    auto D = std::make_unique<DevNet>();
+    D->kind = 1;
+    D->n = (int)norb;
+    D->ecore = energy_nuc;
+    integrals_from_reference_layout((int)norb, one_body_mo, two_body_mo, D->h, D->eri);
+    return D.release();
+    API_CATCH(nullptr)
+}
+
+void *hipotf_build_pool_f64(int64_t norb, int64_t nops, const int32_t *offsets, const int32_t *orbs) {
+    API_TRY
+    HIPOTF_REQUIRE(norb >= 1 && norb <= 32 && nops >= 0, "bad pool sizes");
+    auto D = std::make_unique<DevNet>();
+    D->kind = 2;
+    D->n = (int)norb;
+    D->offsets.assign(offsets, offsets + nops + 1);
+    D->orbs.assign(orbs, orbs + (nops ? offsets[nops] : 0));
+    return D.release();
+    API_CATCH(nullptr)
+}
+
+int hipotf_prepare(void *basis, void *net) {
+    API_TRY
+    DevBasis *B = as_basis(basis);
+    DevNet *N = as_net(net);
+    if (N->kind == 1) N->ham_for(B);
+    else N->pool_for(B);
+    HIPOTF_CHECK(hipDeviceSynchronize());
+    return 0;
+    API_CATCH(1)
+}
+
+int64_t hipotf_num_generators(void *net) {
+    API_TRY
+    DevNet *N = as_net(net);
+    return N->kind == 2 ? (int64_t)N->offsets.size() - 1 : 0;
+    API_CATCH(-1)
+}
+
+void hvec_cuda(void *basis, void *net, const double *src, double *dst) {
+    API_TRY
+    DevBasis *B = as_basis(basis);
+    DevHam *H = as_net(net)->ham_for(B);
+    sigma(H, 1, &src, &dst, 0, S0());
+    API_CATCH_VOID
+}
+
+int hipotf_hvec_block(void *basis, void *net, int nvec, const double *const *src, double *const *dst,
+                      int precision) {
+    API_TRY
+    HIPOTF_REQUIRE(precision == 0 || precision == 1, "precision must be 0 (fp64) or 1 (fp32)");
+    DevBasis *B = as_basis(basis);
+    DevHam *H = as_net(net)->ham_for(B);
+    sigma(H, nvec, src, dst, precision, S0());
+    return 0;
+    API_CATCH(1)
+}
+
+void get_diags_elements_cuda(void *basis, void *net, double *diags) {
+    API_TRY
+    DevBasis *B = as_basis(basis);
+    diag_elements(as_net(net)->ham_for(B), diags, S0());
+    API_CATCH_VOID
+}
+
+void expm_cuda(void *basis, void *net, int64_t idx, double theta, double *vec) {
+    API_TRY
+    DevBasis *B = as_basis(basis);
+    rot_expm(as_net(net)->pool_for(B), idx, theta, vec, S0());
+    API_CATCH_VOID
+}
+
+double grad_cuda(void *basis, void *net, int64_t idx, double theta, const double *lp, const double *rp) {
+    API_TRY
+    DevBasis *B = as_basis(basis);
+    DevPool *P = as_net(net)->pool_for(B);
+    double *d = P->scal.as<double>();
+    rot_grad(P, idx, theta, lp, rp, d, S0());
+    double h = 0.0;
+    HIPOTF_CHECK(hipMemcpy(&h, d, sizeof(double), hipMemcpyDeviceToHost));
+    return h;
+    API_CATCH(kNaN)
+}
+
+double backgrad_cuda(void *basis, void *net, int64_t idx, double theta, double *lp, double *rp) {
+    API_TRY
+    DevBasis *B = as_basis(basis);
+    DevPool *P = as_net(net)->pool_for(B);
+    double *d = P->scal.as<double>();
+    rot_backgrad(P, idx, theta, lp, rp, d, S0());
+    double h = 0.0;
+    HIPOTF_CHECK(hipMemcpy(&h, d, sizeof(double), hipMemcpyDeviceToHost));
+    return h;
+    API_CATCH(kNaN)
+}
+
+void expm_cuda_2d(void *basis, void *net, int64_t idx, double theta, double *vec) {
+    expm_cuda(basis, net, idx, theta, vec);
+}
+double grad_cuda_2d(void *basis, void *net, int64_t idx, double theta, const double *lp, const double *rp) {
+    return grad_cuda(basis, net, idx, theta, lp, rp);
+}
+double backgrad_cuda_2d(void *basis, void *net, int64_t idx, double theta, double *lp, double *rp) {
+    return backgrad_cuda(basis, net, idx, theta, lp, rp);
+}
+
+// grads[g] = grad(g, thetas[g], lp, rp) for every generator; thetas / grads are device arrays
+void batchgrad_cuda(void *basis, void *net, const double *thetas, const double *lp, const double *rp,
+                    double *grads) {
+    API_TRY
+    DevBasis *B = as_basis(basis);
+    DevPool *P = as_net(net)->pool_for(B);
+    HIPOTF_REQUIRE(thetas && grads, "NULL argument");
+    const i64 ng = (i64)P->gens.size();
+    std::vector<double> th(ng);
+    if (ng) HIPOTF_CHECK(hipMemcpy(th.data(), thetas, ng * sizeof(double), hipMemcpyDeviceToHost));
+    for (i64 g = 0; g < ng; ++g) rot_grad(P, g, th[g], lp, rp, grads + g, S0());
+    API_CATCH_VOID
+}
+
+// ---- fused VQE objective (jl/vqe.jl:1-21) ----------------------------------------------------
+// psi: device, in = reference state, out = reference state again (uncomputed by the reverse
+// sweep); lam: device workspace (out = H psi rotated back); grads: host [nparams];
+// out3: host {energy, dH2, |psi|^2}.
+int hipotf_vqe_energy_grad(void *basis, void *ham, void *pool, int64_t nparams, const int64_t *idxs,
+                           const double *thetas, double *psi, double *lam, double *grads, double *out3) {
+    API_TRY
+    DevBasis *B = as_basis(basis);
+    DevHam *H = as_net(ham)->ham_for(B);
+    DevPool *P = as_net(pool)->pool_for(B);
+    HIPOTF_REQUIRE(psi && lam && grads && out3 && (nparams == 0 || (idxs && thetas)), "NULL argument");
+    const i64 dim = B->host.dim;
+    hipStream_t st = S0();
+    for (i64 i = 0; i < nparams; ++i) rot_expm(P, idxs[i], thetas[i], psi, st);
+    const double *src = psi;
+    double *dst = lam;
+    sigma(H, 1, &src, &dst, 0, st);
+    DevBuf dres((size_t)(nparams + 4) * sizeof(double));
+    double *d = dres.as<double>();
+    dot_dev(dim, psi, psi, d + nparams, st);
+    dot_dev(dim, lam, lam, d + nparams + 1, st);
+    dot_dev(dim, psi, lam, d + nparams + 2, st);
+    for (i64 i = nparams - 1; i >= 0; --i) rot_backgrad(P, idxs[i], thetas[i], psi, lam, d + i, st);
+    std::vector<double> h(nparams + 3);
+    HIPOTF_CHECK(hipMemcpyAsync(h.data(), d, (nparams + 3) * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPOTF_CHECK(hipStreamSynchronize(st));
+    const double lnorm = h[nparams], rnorm = h[nparams + 1], e = h[nparams + 2] / lnorm;
+    for (i64 i = 0; i < nparams; ++i) grads[i] = h[i] * 2.0 / lnorm;
+    out3[0] = e;
+    out3[1] = std::max(0.0, rnorm / lnorm - e * e);
+    out3[2] = lnorm;
+    return 0;
+    API_CATCH(1)
+}
+
+// ---- libdiag on the device (src/include/diag/declare.hpp:4-75) -------------------------------
+double build_ritz_and_residual_f64(int64_t N, int dim, const double *coeffs, double min_e,
+                                   const double **V, const double **AV, double *axt) {
+    API_TRY
+    return ritz_and_residual(N, dim, coeffs, min_e, V, AV, axt, S0());
+    API_CATCH(kNaN)
+}
+
+void build_ritz_vector_f64(int64_t N, int dim, const double *coeffs, const double **V, double *xt) {
+    API_TRY
+    ritz_vector(N, dim, coeffs, V, xt, S0());
+    API_CATCH_VOID
+}
+
+void apply_preconditioner_inplace_f64(int64_t N, double *axt, const double *diags, double min_e,
+                                      double shift) {
+    API_TRY
+    precondition(N, axt, diags, min_e, shift, S0());
+    API_CATCH_VOID
+}
+
+double mgs_orthogonalize_f64(int64_t N, int dim, const double **V, double *vt) {
+    API_TRY
+    return mgs(N, dim, V, vt, S0());
+    API_CATCH(kNaN)
+}
+
+double inplace_normalize_f64(int64_t N, double *vec) {
+    API_TRY
+    return normalize(N, vec, S0());
+    API_CATCH(kNaN)
+}
+
+double fast_dot_f64(int64_t N, const double *v1, const double *v2) {
+    API_TRY
+    return dot(N, v1, v2, S0());
+    API_CATCH(kNaN)
+}
+
+void compute_heff_col_f64(int64_t N, int dim, const double **V, const double *AV_dim, double *out_col) {
+    API_TRY
+    multidot(N, dim, V, AV_dim, out_col, S0());
+    API_CATCH_VOID
+}
+
+void rebuild_heff_f64(int64_t N, int dim, const double **V, const double **AV, double *heff, int ldh) {
+    API_TRY
+    std::vector<double> col(dim);
+    for (int c = 0; c < dim; ++c) {
+        multidot(N, dim, V, AV[c], col.data(), S0());
+        for (int r = 0; r < dim; ++r) heff[r + (size_t)c * ldh] = col[r];
+    }
+    API_CATCH_VOID
+}
+
+}  // extern "C"

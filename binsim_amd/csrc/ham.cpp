@@ -1,0 +1,185 @@
+// Two-body Hamiltonian in integral form, bound to a device basis.
+//
+// The operator is the one the reference builds as Pauli terms in
+// generate_hamiltonian_real_tmpl (src/include/ham/ham.hpp:998-1069):
+//   H = E_core + sum_pq h_pq sum_s a+_ps a_qs + 1/2 sum_pqrs (ps|qr) sum_st a+_ps a+_qt a_rt a_ss
+// with the interleaved Jordan-Wigner order.  Here it is split as
+//   H = E ( G_alpha (x) 1 + 1 (x) G_beta + H_alpha-beta + E_core ) E
+// where E is the diagonal +-1 gauge from interleaved to alpha-major determinant order and
+//   G[I,I]   = sum_{p in I} h_pp + 1/2 sum_{p,q in I} [(pp|qq) - (pq|qp)]
+//   G[I,J]   = sgn [h_rs + sum_{j in I, j != r} ((rs|jj) - (rj|js))]        J = I - r + s
+//   G[I,J]   = sgn1 sgn2 [(r1 s1|r2 s2) - (r1 s2|r2 s1)]                    J = I - r1 - r2 + s1 + s2
+//   H_ab     = sum_pqrs (pq|rs) E^a_pq E^b_rs     (p = q and r = s terms included)
+// Integrals must carry the 8-fold permutational symmetry of real orbitals (checked).
+#include "hipotf.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <thread>
+
+namespace hipotf {
+
+i64 host_find_str(const SpinSpace &S, u64 s);
+
+static inline int popc64(u64 v) { return __builtin_popcountll(v); }
+static inline u64 between_mask(int r, int s) {
+    int lo = r < s ? r : s, hi = r < s ? s : r;
+    return ((1ull << hi) - 1) & ~((1ull << (lo + 1)) - 1);
+}
+
+template <typename F>
+static void par_for(i64 n, F f) {
+    unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+    if (n < 1024 || nt == 1) {
+        for (i64 i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t)
+        th.emplace_back([=, &f]() {
+            for (i64 i = t; i < n; i += nt) f(i);
+        });
+    for (auto &x : th) x.join();
+}
+
+static void build_same_spin(SameSpinTable &T, const SpinSpace &S, const std::vector<double> &h,
+                            const std::vector<double> &eri, int n) {
+    const int k = S.k, nv = n - k;
+    T.L = 1 + S.nsx + S.ndx;
+    const size_t L = T.L;
+    std::vector<int32_t> J((size_t)S.N * L);
+    std::vector<double> C((size_t)S.N * L);
+    std::vector<double> gd(S.N);
+    auto E = [&](int p, int q, int r, int s) { return eri[(((size_t)p * n + q) * n + r) * n + s]; };
+    par_for(S.N, [&](i64 I) {
+        const u64 s = S.strs[I];
+        int occ[64], vir[64], no = 0, nvv = 0;
+        for (int p = 0; p < n; ++p) {
+            if ((s >> p) & 1) occ[no++] = p;
+            else vir[nvv++] = p;
+        }
+        int32_t *Jr = &J[(size_t)I * L];
+        double *Cr = &C[(size_t)I * L];
+        double d = 0.0;
+        for (int a = 0; a < k; ++a) {
+            int p = occ[a];
+            d += h[p * n + p];
+            for (int b = 0; b < k; ++b) {
+                int q = occ[b];
+                d += 0.5 * (E(p, p, q, q) - E(p, q, q, p));
+            }
+        }
+        gd[I] = d;
+        Jr[0] = (int32_t)I;
+        Cr[0] = d;
+        size_t l = 1;
+        for (int io = 0; io < k; ++io)
+            for (int iv = 0; iv < nv; ++iv, ++l) {
+                int r = occ[io], t = vir[iv];
+                u64 js = s ^ (1ull << r) ^ (1ull << t);
+                double v = h[r * n + t];
+                for (int a = 0; a < k; ++a) {
+                    int j = occ[a];
+                    if (j == r) continue;
+                    v += E(r, t, j, j) - E(r, j, j, t);
+                }
+                bool neg = popc64(s & between_mask(r, t)) & 1;
+                Jr[l] = (int32_t)host_find_str(S, js);
+                Cr[l] = neg ? -v : v;
+            }
+        for (int a1 = 0; a1 < k; ++a1)
+            for (int a2 = a1 + 1; a2 < k; ++a2)
+                for (int b1 = 0; b1 < nv; ++b1)
+                    for (int b2 = b1 + 1; b2 < nv; ++b2, ++l) {
+                        int r1 = occ[a1], r2 = occ[a2], s1 = vir[b1], s2 = vir[b2];
+                        u64 js = s ^ (1ull << r1) ^ (1ull << r2) ^ (1ull << s1) ^ (1ull << s2);
+                        // apply E_{r1 s1} then E_{r2 s2} to J to reach I
+                        int g1 = popc64(js & between_mask(r1, s1)) & 1;
+                        u64 j1 = js ^ (1ull << s1) ^ (1ull << r1);
+                        int g2 = popc64(j1 & between_mask(r2, s2)) & 1;
+                        double v = E(r1, s1, r2, s2) - E(r1, s2, r2, s1);
+                        Jr[l] = (int32_t)host_find_str(S, js);
+                        Cr[l] = (g1 ^ g2) ? -v : v;
+                    }
+    });
+    for (size_t i = 0; i < J.size(); ++i)
+        HIPOTF_REQUIRE(J[i] >= 0 && J[i] < S.N, "internal: same-spin table index out of range");
+    T.J.upload(J);
+    T.c64.upload(C);
+    std::vector<float> C32(C.begin(), C.end());
+    T.c32.upload(C32);
+    T.gdiag.upload(gd);
+}
+
+void integrals_from_reference_layout(int n, const double *h_cm, const double *g_ref,
+                                     std::vector<double> &h, std::vector<double> &eri) {
+    HIPOTF_REQUIRE(n >= 1 && n <= 32, "norb must be in [1, 32]");
+    HIPOTF_REQUIRE(h_cm && g_ref, "integral pointers are NULL");
+    const size_t n2 = (size_t)n * n, n3 = n2 * n, n4 = n3 * n;
+    h.resize(n2);
+    eri.resize(n4);
+    for (int p = 0; p < n; ++p)
+        for (int q = 0; q < n; ++q) h[p * n + q] = h_cm[p + (size_t)q * n];
+    // (pq|rs) = two_body_mo[p, r, s, q]  (two_body_mo[p,q,r,s] = (ps|qr), column-major)
+    for (int p = 0; p < n; ++p)
+        for (int q = 0; q < n; ++q)
+            for (int r = 0; r < n; ++r)
+                for (int s = 0; s < n; ++s)
+                    eri[((size_t)(p * n + q) * n + r) * n + s] =
+                        g_ref[p + (size_t)r * n + (size_t)s * n2 + (size_t)q * n3];
+    double scale = 0.0;
+    for (double v : eri) scale = std::max(scale, std::fabs(v));
+    for (double v : h) scale = std::max(scale, std::fabs(v));
+    const double tol = 1e-10 * std::max(1.0, scale);
+    for (int p = 0; p < n; ++p)
+        for (int q = 0; q < n; ++q) {
+            HIPOTF_REQUIRE(std::fabs(h[p * n + q] - h[q * n + p]) <= tol,
+                           "one-electron integrals are not symmetric");
+            for (int r = 0; r < n; ++r)
+                for (int s = 0; s < n; ++s) {
+                    const double v = eri[((size_t)(p * n + q) * n + r) * n + s];
+                    const double v1 = eri[((size_t)(q * n + p) * n + r) * n + s];
+                    const double v2 = eri[((size_t)(r * n + s) * n + p) * n + q];
+                    HIPOTF_REQUIRE(std::fabs(v - v1) <= tol && std::fabs(v - v2) <= tol,
+                                   "two-electron integrals lack 8-fold symmetry");
+                }
+        }
+}
+
+DevHam *make_dev_ham(DevBasis *b, int n, const std::vector<double> &h, const std::vector<double> &eri,
+                     double ecore) {
+    HIPOTF_REQUIRE(b != nullptr, "basis handle is NULL");
+    HIPOTF_REQUIRE(n == b->host.norb, "orbital count of the operator does not match the basis");
+    auto H = std::make_unique<DevHam>();
+    H->basis = b;
+    H->n = n;
+    H->ecore = ecore;
+    H->h = h;
+    H->eri = eri;
+    H->eri64.upload(H->eri);
+    std::vector<float> e32(H->eri.begin(), H->eri.end());
+    H->eri32.upload(e32);
+    const HostBasis &hb = b->host;
+    build_same_spin(H->tA, hb.A, H->h, H->eri, n);
+    H->ptA = &H->tA;
+    if (b->pB == b->pA) {
+        H->ptB = &H->tA;
+    } else {
+        build_same_spin(H->tB, hb.B, H->h, H->eri, n);
+        H->ptB = &H->tB;
+    }
+    std::vector<double> jab((size_t)hb.A.N * n, 0.0);
+    for (i64 a = 0; a < hb.A.N; ++a) {
+        u64 s = hb.A.strs[a];
+        for (int r = 0; r < n; ++r) {
+            double v = 0.0;
+            for (int p = 0; p < n; ++p)
+                if ((s >> p) & 1) v += H->eri[((size_t)(p * n + p) * n + r) * n + r];
+            jab[(size_t)a * n + r] = v;
+        }
+    }
+    H->jab.upload(jab);
+    return H.release();
+}
+
+}  // namespace hipotf

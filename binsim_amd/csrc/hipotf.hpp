@@ -1,0 +1,212 @@
+// Internal declarations of libhipotf (not part of the C ABI; see include/hipotf.h).
+//
+// Data layout in HBM (one symmetry block, |a,b> at index ia*Nb + ib as in
+// src/include/basis/basis.hpp:327):
+//   * state vectors: float64[dim], alpha-string-major (a row = one alpha string, contiguous
+//     beta strings), interleaved Jordan-Wigner sign convention of the reference
+//     (spin-orbital 2p = alpha_p, 2p+1 = beta_p; src/include/ham/ham.hpp:419-451).
+//   * sigma work buffers: T[dim*nv] with the nv vectors of a block interleaved per
+//     determinant, in the alpha-major gauge  x~ = E x,  E_ab = (-1)^popc(b & M(a)),
+//     M(a) = XOR_{p in a} ((1<<p)-1); both alpha-major and beta-major copies.
+//   * string tables per spin: colex-ordered strings, single-excitation table SX[slot][I]
+//     (packed J|r|s|sign), and the same-spin Hamiltonian G as fixed-length CSR rows.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace hipotf {
+
+using i64 = int64_t;
+using u32 = uint32_t;
+using u64 = uint64_t;
+
+// ------------------------------------------------------------------------------------------
+// errors: every C entry point catches, records a thread-local message and returns a sentinel
+// ------------------------------------------------------------------------------------------
+void set_last_error(const std::string &msg);
+void clear_last_error();
+
+struct Error : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIPOTF_CHECK(expr)                                                                   \
+    do {                                                                                     \
+        hipError_t _e = (expr);                                                              \
+        if (_e != hipSuccess)                                                                \
+            throw ::hipotf::Error(std::string(#expr) + " failed: " + hipGetErrorString(_e)); \
+    } while (0)
+
+#define HIPOTF_REQUIRE(cond, msg)                   \
+    do {                                            \
+        if (!(cond)) throw ::hipotf::Error(msg);    \
+    } while (0)
+
+// RAII device allocation
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { alloc(n); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t n) {
+        release();
+        if (n == 0) return;
+        HIPOTF_CHECK(hipMalloc(&p, n));
+        bytes = n;
+    }
+    void ensure(size_t n) { if (n > bytes) alloc(n); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T> T *as() const { return static_cast<T *>(p); }
+    template <typename T> void upload(const std::vector<T> &v) {
+        alloc(v.size() * sizeof(T));
+        if (!v.empty()) HIPOTF_CHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    }
+};
+
+// ------------------------------------------------------------------------------------------
+// host-side basis (mirror of BasisManager, src/include/basis/basis.hpp:29-110)
+// ------------------------------------------------------------------------------------------
+struct SpinSpace {
+    int n = 0, k = 0;
+    i64 N = 0;                       // C(n,k)
+    std::vector<u64> strs;           // ascending numeric (= colex) order
+    int nsx = 0;                     // k*(n-k) single excitations per string
+    int ndx = 0;                     // C(k,2)*C(n-k,2) double excitations per string
+};
+
+struct HostBasis {
+    i64 norb = 0, na = 0, nb = 0, total_sym = 0, num_irreps = 0;
+    std::vector<i64> orbsym;
+    i64 dim = 0;
+    i64 num_blocks = 0;
+    SpinSpace A, B;
+};
+
+HostBasis *make_host_basis(i64 norb, i64 na, i64 nb, i64 total_sym, const i64 *orbsym, i64 num_irreps);
+
+// single-excitation table entry: J (20 bits) | r (5) | s (5) | unused | sign (bit 31)
+// meaning: dest string I has orbital r occupied, s empty; J = I - r + s; <I|E_rs|J> = sign.
+__host__ __device__ inline u32 sx_pack(u32 J, int r, int s, bool neg) {
+    return J | (u32(r) << 20) | (u32(s) << 25) | (neg ? 0x80000000u : 0u);
+}
+__host__ __device__ inline u32 sx_J(u32 e) { return e & 0xFFFFFu; }
+__host__ __device__ inline int sx_r(u32 e) { return int((e >> 20) & 31u); }
+__host__ __device__ inline int sx_s(u32 e) { return int((e >> 25) & 31u); }
+__host__ __device__ inline bool sx_neg(u32 e) { return (e >> 31) != 0u; }
+
+// ------------------------------------------------------------------------------------------
+// device basis (counterpart of BasisViewDev, src/include/cuda/basis.cuh:8-186)
+// ------------------------------------------------------------------------------------------
+struct DevSpin {
+    int n = 0, k = 0, nsx = 0, ndx = 0;
+    i64 N = 0;
+    DevBuf strs;      // u64[N]
+    DevBuf sx;        // u32[nsx][N]
+};
+
+struct DevBasis {
+    HostBasis host;           // own copy of the host description
+    DevSpin A, B;             // B aliases A's tables when na == nb (shared pointers below)
+    DevSpin *pA = nullptr, *pB = nullptr;
+    DevBuf gaugeM;            // u64[Na]: M(a) = XOR_{p in a} ((1<<p)-1)
+    i64 uid = 0;              // unique id (operators cache per-basis tables under it)
+    // scratch shared by sigma calls on this basis
+    DevBuf xt, xtT, s1, s2T;
+    DevBuf red;               // reduction partials
+};
+
+DevBasis *make_dev_basis(const HostBasis &hb);
+
+// ------------------------------------------------------------------------------------------
+// Hamiltonian in integral form (replaces the SVD network of src/include/ham/otf.hpp:114-169
+// for the 2-body H; the network constructor decodes a network into this form)
+// ------------------------------------------------------------------------------------------
+struct SameSpinTable {
+    int L = 0;                // 1 + nsx + ndx entries per dest string (diagonal first)
+    DevBuf J;                 // int32[N][L]
+    DevBuf c64, c32;          // double / float [N][L]
+    DevBuf gdiag;             // double[N]  same-spin diagonal G[I,I]
+};
+
+struct DevHam {
+    DevBasis *basis = nullptr;
+    int n = 0;
+    double ecore = 0.0;
+    std::vector<double> h, eri;   // host copies: h[p*n+q], eri[((p*n+q)*n+r)*n+s] = (pq|rs)
+    DevBuf eri64, eri32;          // (pq|rs) on device
+    DevBuf jab;                   // double[Na][n]: sum_{p in a} (pp|rr)
+    SameSpinTable tA, tB;
+    SameSpinTable *ptA = nullptr, *ptB = nullptr;
+};
+
+void integrals_from_reference_layout(int n, const double *h_colmajor, const double *g_ref_colmajor,
+                                     std::vector<double> &h, std::vector<double> &eri);
+DevHam *make_dev_ham(DevBasis *b, int n, const std::vector<double> &h, const std::vector<double> &eri,
+                     double ecore);
+
+// sigma: dst_v = H src_v for nv vectors; prec 0 = float64 work, 1 = float32 work
+void sigma(DevHam *H, int nv, const double *const *src, double *const *dst, int prec,
+           hipStream_t st);
+void diag_elements(DevHam *H, double *diags, hipStream_t st);  // accumulates
+
+// ------------------------------------------------------------------------------------------
+// generator pool (UCC excitations G = T - T^dagger, jl/ansatz.jl:230-276)
+// ------------------------------------------------------------------------------------------
+struct PairList {            // pattern within one spin: strings u with ann subset, cre disjoint
+    u64 cre = 0, ann = 0;
+    i64 len = 0;
+    DevBuf uw;               // int32[2*len]: (index of u, index of w = u^cre^ann)
+    bool identity = false;   // (x, x) for every string
+    int spin = 0;            // 0 alpha, 1 beta
+};
+
+struct Generator {
+    int nops = 0;
+    int pos[8];              // interleaved spin-orbital positions, in APPLICATION order
+    int cre[8];              // 1 = creation
+    int kind = 0;            // 0 identity, 1 alpha only, 2 beta only, 3 mixed
+    int pa = -1, pb = -1;    // pattern ids (into DevPool::lists)
+};
+
+struct DevPool {
+    DevBasis *basis = nullptr;
+    std::vector<Generator> gens;
+    std::vector<PairList> lists;   // lists[0] = alpha identity, lists[1] = beta identity
+    DevBuf partial;                // double partial sums for reductions
+    DevBuf scal;                   // a few device doubles
+};
+
+DevPool *make_dev_pool(DevBasis *b, i64 nops, const int32_t *offsets, const int32_t *orbs);
+
+void rot_expm(DevPool *P, i64 idx, double theta, double *vec, hipStream_t st);
+// grad: returns device pointer to the scalar result (written asynchronously)
+void rot_grad(DevPool *P, i64 idx, double theta, const double *lp, const double *rp, double *dres,
+              hipStream_t st);
+void rot_backgrad(DevPool *P, i64 idx, double theta, double *lp, double *rp, double *dres,
+                  hipStream_t st);
+
+// device reductions used by Davidson / VQE
+void dev_dot(const double *a, const double *b, i64 n, double *dres, DevBuf &scratch, hipStream_t st);
+
+int num_cus();
+
+}  // namespace hipotf

@@ -1,0 +1,329 @@
+// Givens-rotation operators of a UCC generator pool: expm / grad / backgrad.
+//
+// Replaces expm_svd_network_otf_gpu, grad_svd_network_otf_gpu and
+// backgrad_svd_network_otf_gpu (src/include/cuda/expm.cuh:141-197, grad.cuh:144-203,
+// backgrad.cuh:152-212) and their CPU counterparts (src/include/op/expm.hpp:374-438,
+// grad.hpp:397-465, backgrad.hpp:406-474).
+//
+// A generator G = T - T^dagger with T = a+_{c1} .. a+_{cm} a_{a1} .. a_{am} (interleaved
+// spin-orbitals, jl/ansatz.jl:230-250) couples each determinant u that T maps to a nonzero
+// state with w = T u / t(u), t(u) = +-1 its Jordan-Wigner sign.  Instead of scanning every
+// determinant and evaluating the SVD phase network, the kernels walk exactly those pairs:
+// per spin a pair list (u index, w index) of the strings matching the generator's pattern
+// (identity list for a spin the generator does not touch), and t(u) is computed on the fly
+// by applying the ladder operators to (a, b) with popcounts.  Per pair, with c = cos(theta),
+// s = sin(theta) (update formulas of src/include/core/math.hpp:132-215, |vt| = 1):
+//   expm     : x_u <- c x_u - t s x_w ,  x_w <- t s x_u + c x_w
+//   grad     : g += l_u(-s r_u + t c r_w) + l_w(-s r_w - t c r_u)
+//   backgrad : l <- R(-theta) l ; g += [same form with new l, old r] ; r <- R(-theta) r
+#include "hipotf.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace hipotf {
+
+i64 host_find_str(const SpinSpace &S, u64 s);
+
+struct GenDev {
+    int nops;
+    int pos[8];
+    int cre[8];
+};
+
+__device__ __forceinline__ int jw_sign(const GenDev &g, u64 a, u64 b) {
+    int par = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        if (i >= g.nops) break;
+        const int p = g.pos[i];
+        const int q = p >> 1;
+        const u64 lowq = (1ull << q) - 1;
+        if ((p & 1) == 0) {
+            par += __popcll(a & lowq) + __popcll(b & lowq);
+            a ^= 1ull << q;
+        } else {
+            par += __popcll(a & ((lowq << 1) | 1ull)) + __popcll(b & lowq);
+            b ^= 1ull << q;
+        }
+    }
+    return par & 1;
+}
+
+struct PairView {
+    const int32_t *la, *lb;   // (u, w) index pairs
+    i64 na, nb;               // list lengths
+    i64 NB;                   // beta strings (row length)
+    const u64 *strsA, *strsB;
+};
+
+__device__ __forceinline__ void pair_at(const PairView &v, i64 i, i64 &u, i64 &w, u64 &ua, u64 &ub) {
+    const i64 ia = i / v.nb, ib = i - ia * v.nb;
+    const int32_t uA = v.la[2 * ia], wA = v.la[2 * ia + 1];
+    const int32_t uB = v.lb[2 * ib], wB = v.lb[2 * ib + 1];
+    u = (i64)uA * v.NB + uB;
+    w = (i64)wA * v.NB + wB;
+    ua = v.strsA[uA];
+    ub = v.strsB[uB];
+}
+
+__global__ __launch_bounds__(256) void k_expm(PairView v, GenDev g, double c, double s,
+                                              double *__restrict__ x) {
+    const i64 tot = v.na * v.nb;
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (i64)gridDim.x * blockDim.x) {
+        i64 u, w;
+        u64 ua, ub;
+        pair_at(v, i, u, w, ua, ub);
+        const double ts = jw_sign(g, ua, ub) ? -s : s;
+        const double xu = x[u], xw = x[w];
+        x[u] = c * xu - ts * xw;
+        x[w] = ts * xu + c * xw;
+    }
+}
+
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double red[16];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int i = 0; i < nw; ++i) t += red[i];
+    }
+    return t;  // valid in thread 0
+}
+
+__global__ __launch_bounds__(256) void k_grad(PairView v, GenDev g, double c, double s,
+                                              const double *__restrict__ l, const double *__restrict__ r,
+                                              double *__restrict__ partial) {
+    const i64 tot = v.na * v.nb;
+    double acc = 0.0;
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (i64)gridDim.x * blockDim.x) {
+        i64 u, w;
+        u64 ua, ub;
+        pair_at(v, i, u, w, ua, ub);
+        const double tc = jw_sign(g, ua, ub) ? -c : c;
+        const double ru = r[u], rw = r[w];
+        acc += l[u] * (-s * ru + tc * rw) + l[w] * (-s * rw - tc * ru);
+    }
+    const double t = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(256) void k_backgrad(PairView v, GenDev g, double c, double s,
+                                                  double *__restrict__ l, double *__restrict__ r,
+                                                  double *__restrict__ partial) {
+    const i64 tot = v.na * v.nb;
+    double acc = 0.0;
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (i64)gridDim.x * blockDim.x) {
+        i64 u, w;
+        u64 ua, ub;
+        pair_at(v, i, u, w, ua, ub);
+        const bool neg = jw_sign(g, ua, ub);
+        const double ts = neg ? -s : s, tc = neg ? -c : c;
+        // R(-theta): x_u <- c x_u + t s x_w ; x_w <- -t s x_u + c x_w
+        const double lu = l[u], lw = l[w], ru = r[u], rw = r[w];
+        const double nlu = c * lu + ts * lw, nlw = -ts * lu + c * lw;
+        acc += nlu * (-s * ru + tc * rw) + nlw * (-s * rw - tc * ru);
+        l[u] = nlu;
+        l[w] = nlw;
+        r[u] = c * ru + ts * rw;
+        r[w] = -ts * ru + c * rw;
+    }
+    const double t = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// sum of partial[0..n) into *out (+ optional scale) by one block, fixed order -> deterministic
+__global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__ partial, int n,
+                                                      double scale, double *__restrict__ out) {
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partial[i];
+    const double t = block_sum(acc);
+    if (threadIdx.x == 0) *out = t * scale;
+}
+
+// ---------------------------------------------------------------------------------------
+// pool construction
+// ---------------------------------------------------------------------------------------
+static int pattern_id(DevPool &P, const SpinSpace &S, int spin, u64 cre, u64 ann) {
+    if (cre == 0 && ann == 0) return spin;  // identity list of that spin
+    for (size_t i = 2; i < P.lists.size(); ++i)
+        if (P.lists[i].spin == spin && P.lists[i].cre == cre && P.lists[i].ann == ann) return (int)i;
+    PairList L;
+    L.cre = cre;
+    L.ann = ann;
+    L.spin = spin;
+    std::vector<int32_t> uw;
+    for (i64 i = 0; i < S.N; ++i) {
+        const u64 s = S.strs[i];
+        if ((s & ann) == ann && (s & cre) == 0) {
+            const i64 j = host_find_str(S, s ^ ann ^ cre);
+            HIPOTF_REQUIRE(j >= 0, "internal: excited string missing from the basis");
+            uw.push_back((int32_t)i);
+            uw.push_back((int32_t)j);
+        }
+    }
+    L.len = (i64)(uw.size() / 2);
+    L.uw.upload(uw);
+    P.lists.push_back(std::move(L));
+    return (int)P.lists.size() - 1;
+}
+
+DevPool *make_dev_pool(DevBasis *b, i64 nops, const int32_t *offsets, const int32_t *orbs) {
+    HIPOTF_REQUIRE(b != nullptr, "basis handle is NULL");
+    HIPOTF_REQUIRE(nops >= 0 && (nops == 0 || (offsets && orbs)), "bad pool arrays");
+    auto P = std::make_unique<DevPool>();
+    P->basis = b;
+    const HostBasis &hb = b->host;
+    const int n = (int)hb.norb;
+    // identity lists
+    for (int sp = 0; sp < 2; ++sp) {
+        const SpinSpace &S = sp == 0 ? hb.A : hb.B;
+        PairList L;
+        L.identity = true;
+        L.spin = sp;
+        std::vector<int32_t> uw(2 * S.N);
+        for (i64 i = 0; i < S.N; ++i) uw[2 * i] = uw[2 * i + 1] = (int32_t)i;
+        L.len = S.N;
+        L.uw.upload(uw);
+        P->lists.push_back(std::move(L));
+    }
+    for (i64 g = 0; g < nops; ++g) {
+        const int32_t o0 = offsets[g], o1 = offsets[g + 1];
+        const int len = o1 - o0;
+        HIPOTF_REQUIRE(len >= 2 && len <= 8 && len % 2 == 0,
+                       "generator " + std::to_string(g) + ": need 2..8 spin-orbitals, creators first");
+        Generator G;
+        G.nops = len;
+        u64 cre_a = 0, ann_a = 0, cre_b = 0, ann_b = 0;
+        // T = a+_{l0} .. a+_{l(mid-1)} a_{l(mid)} .. a_{l(len-1)}; applied right to left
+        for (int i = 0; i < len; ++i) {
+            const int pos = orbs[o0 + i];
+            HIPOTF_REQUIRE(pos >= 0 && pos < 2 * n, "generator spin-orbital out of range");
+            const bool is_cre = i < len / 2;
+            const u64 bit = 1ull << (pos >> 1);
+            u64 &m = (pos & 1) ? (is_cre ? cre_b : ann_b) : (is_cre ? cre_a : ann_a);
+            HIPOTF_REQUIRE((m & bit) == 0, "generator repeats a spin-orbital (T = 0)");
+            m |= bit;
+            G.pos[len - 1 - i] = pos;
+            G.cre[len - 1 - i] = is_cre ? 1 : 0;
+        }
+        HIPOTF_REQUIRE((cre_a & ann_a) == 0 && (cre_b & ann_b) == 0,
+                       "number-operator (diagonal) generators are not supported");
+        HIPOTF_REQUIRE(__builtin_popcountll(cre_a) == __builtin_popcountll(ann_a) &&
+                           __builtin_popcountll(cre_b) == __builtin_popcountll(ann_b),
+                       "generator does not conserve N_alpha and N_beta");
+        const bool ta = cre_a != 0, tb = cre_b != 0;
+        G.kind = ta && tb ? 3 : (ta ? 1 : 2);
+        G.pa = pattern_id(*P, hb.A, 0, cre_a, ann_a);
+        G.pb = pattern_id(*P, hb.B, 1, cre_b, ann_b);
+        P->gens.push_back(G);
+    }
+    P->partial.alloc(4096 * sizeof(double));
+    P->scal.alloc(64 * sizeof(double));
+    return P.release();
+}
+
+static void views(DevPool *P, i64 idx, PairView &v, GenDev &gd) {
+    HIPOTF_REQUIRE(P != nullptr, "pool handle is NULL");
+    HIPOTF_REQUIRE(idx >= 0 && idx < (i64)P->gens.size(),
+                   "generator index " + std::to_string(idx) + " out of range");
+    const Generator &G = P->gens[idx];
+    DevBasis *B = P->basis;
+    const PairList &LA = P->lists[G.pa], &LB = P->lists[G.pb];
+    v.la = LA.uw.as<int32_t>();
+    v.lb = LB.uw.as<int32_t>();
+    v.na = LA.len;
+    v.nb = LB.len;
+    v.NB = B->pB->N;
+    v.strsA = B->pA->strs.as<u64>();
+    v.strsB = B->pB->strs.as<u64>();
+    gd.nops = G.nops;
+    for (int i = 0; i < 8; ++i) {
+        gd.pos[i] = i < G.nops ? G.pos[i] : 0;
+        gd.cre[i] = i < G.nops ? G.cre[i] : 0;
+    }
+}
+
+static int grid_for(i64 tot) {
+    const i64 want = (tot + 255) / 256;
+    return (int)std::max<i64>(1, std::min<i64>(want, 2048));
+}
+
+void rot_expm(DevPool *P, i64 idx, double theta, double *vec, hipStream_t st) {
+    PairView v;
+    GenDev g;
+    views(P, idx, v, g);
+    const i64 tot = v.na * v.nb;
+    if (tot == 0) return;
+    HIPOTF_REQUIRE(vec != nullptr, "NULL vector");
+    hipLaunchKernelGGL(k_expm, dim3(grid_for(tot)), dim3(256), 0, st, v, g, std::cos(theta),
+                       std::sin(theta), vec);
+    HIPOTF_CHECK(hipGetLastError());
+}
+
+void rot_grad(DevPool *P, i64 idx, double theta, const double *lp, const double *rp, double *dres,
+              hipStream_t st) {
+    PairView v;
+    GenDev g;
+    views(P, idx, v, g);
+    const i64 tot = v.na * v.nb;
+    HIPOTF_REQUIRE(lp && rp && dres, "NULL vector");
+    const int nb = tot ? grid_for(tot) : 1;
+    if (tot) {
+        hipLaunchKernelGGL(k_grad, dim3(nb), dim3(256), 0, st, v, g, std::cos(theta), std::sin(theta),
+                           lp, rp, P->partial.as<double>());
+        HIPOTF_CHECK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, P->partial.as<double>(),
+                       tot ? nb : 0, 1.0, dres);
+    HIPOTF_CHECK(hipGetLastError());
+}
+
+void rot_backgrad(DevPool *P, i64 idx, double theta, double *lp, double *rp, double *dres,
+                  hipStream_t st) {
+    PairView v;
+    GenDev g;
+    views(P, idx, v, g);
+    const i64 tot = v.na * v.nb;
+    HIPOTF_REQUIRE(lp && rp && dres, "NULL vector");
+    const int nb = tot ? grid_for(tot) : 1;
+    if (tot) {
+        hipLaunchKernelGGL(k_backgrad, dim3(nb), dim3(256), 0, st, v, g, std::cos(theta),
+                           std::sin(theta), lp, rp, P->partial.as<double>());
+        HIPOTF_CHECK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, P->partial.as<double>(),
+                       tot ? nb : 0, 1.0, dres);
+    HIPOTF_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------
+// dot product (two-stage, deterministic)
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_dot_partial(const double *__restrict__ a,
+                                                     const double *__restrict__ b, i64 n,
+                                                     double *__restrict__ partial) {
+    double acc = 0.0;
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
+        acc += a[i] * b[i];
+    const double t = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+void dev_dot(const double *a, const double *b, i64 n, double *dres, DevBuf &scratch, hipStream_t st) {
+    const int nb = (int)std::max<i64>(1, std::min<i64>((n + 255) / 256, 2048));
+    scratch.ensure(2048 * sizeof(double));
+    if (n > 0) {
+        hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(256), 0, st, a, b, n, scratch.as<double>());
+        HIPOTF_CHECK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, scratch.as<double>(), n > 0 ? nb : 0,
+                       1.0, dres);
+    HIPOTF_CHECK(hipGetLastError());
+}
+
+}  // namespace hipotf

@@ -1,0 +1,429 @@
+// Sigma build  sigma = H x  on gfx950, excitation-driven (no Pauli network scan).
+//
+// Replaces cuda_hvec / hvec_gather_*_kernel (src/include/cuda/hvec.cuh:6-446) and the CPU
+// contract_network_otf (src/include/op/hvec.hpp:481-496).  Pipeline per block of nv vectors:
+//   K1 k_gauge_in   x -> x~ = E x, written alpha-major (xt) and beta-major (xtT), T precision
+//   K2 k_rowspmm    s1  = G_alpha x~          rows = alpha strings, coalesced row streams
+//   K3 k_rowspmm    s2T = G_beta  x~^T        same kernel on the beta-major copy
+//   K4 k_ab         s1 += H_ab x~              one workgroup per destination alpha row; each
+//                                               source row staged in LDS, beta connectivity
+//                                               read per lane, integrals (pq|rs) row in LDS
+//   K5 k_finalize   sigma = E (s1 + s2T^T + E_core x~)   tiled transpose, FP64 output
+// T = double (drop-in FP64) or float (mixed precision).  Sum order differs from the
+// reference, so FP64 results agree to rounding (~1e-15 relative), not bit-for-bit.
+#include "hipotf.hpp"
+
+#include <algorithm>
+
+namespace hipotf {
+
+constexpr int MAXNV = 8;
+struct SrcPack { const double *p[MAXNV]; };
+struct DstPack { double *p[MAXNV]; };
+
+__device__ __forceinline__ int parity64(u64 v) { return __popcll(v) & 1; }
+
+// ---------------------------------------------------------------------------------------
+// K1: gauge + dual layout (32x32 tiles through LDS)
+// ---------------------------------------------------------------------------------------
+template <typename T, int NV>
+__global__ __launch_bounds__(256) void k_gauge_in(SrcPack src, T *__restrict__ xt, T *__restrict__ xtT,
+                                                  const u64 *__restrict__ strsB,
+                                                  const u64 *__restrict__ M, i64 NA, i64 NB) {
+    __shared__ T tile[32][33];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    const i64 b0 = (i64)blockIdx.x * 32, a0 = (i64)blockIdx.y * 32;
+#pragma unroll 1
+    for (int v = 0; v < NV; ++v) {
+        for (int i = 0; i < 4; ++i) {
+            const i64 a = a0 + ty + 8 * i, b = b0 + tx;
+            if (a < NA && b < NB) {
+                double val = src.p[v][a * NB + b];
+                if (parity64(strsB[b] & M[a])) val = -val;
+                const T tv = (T)val;
+                xt[(a * NB + b) * NV + v] = tv;
+                tile[ty + 8 * i][tx] = tv;
+            }
+        }
+        __syncthreads();
+        for (int i = 0; i < 4; ++i) {
+            const i64 b = b0 + ty + 8 * i, a = a0 + tx;
+            if (a < NA && b < NB) xtT[(b * NA + a) * NV + v] = tile[tx][ty + 8 * i];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// K2/K3: Y[I, :] = sum_l C[I,l] X[J[I,l], :]   (fixed-length CSR rows, uniform per wave)
+// One wave = one destination row x 64 consecutive (flattened) columns.  Blocks are mapped so
+// that each XCD walks all rows of one column tile before the next, keeping the tile's
+// source columns resident in that XCD's L2.
+// ---------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_rowspmm(const T *__restrict__ X, T *__restrict__ Y,
+                                                 const int32_t *__restrict__ J,
+                                                 const T *__restrict__ C, int L, i64 nrows,
+                                                 i64 ncols, i64 ntiles, i64 bpt) {
+    const int lane = threadIdx.x & 63;
+    const i64 b = blockIdx.x;
+    const i64 xcd = b & 7, j = b >> 3;
+    const i64 tile = (j / bpt) * 8 + xcd;
+    const i64 rb = j % bpt;
+    if (tile >= ntiles) return;
+    const i64 row = __builtin_amdgcn_readfirstlane((int)(rb * 4 + (threadIdx.x >> 6)));
+    if (row >= nrows) return;
+    const i64 col = tile * 64 + lane;
+    const bool ok = col < ncols;
+    const int32_t *Jr = J + row * (i64)L;
+    const T *Cr = C + row * (i64)L;
+    T acc0 = 0, acc1 = 0;
+    int l = 0;
+    if (ok) {
+        for (; l + 4 <= L; l += 4) {
+            const int j0 = Jr[l], j1 = Jr[l + 1], j2 = Jr[l + 2], j3 = Jr[l + 3];
+            const T x0 = X[(i64)j0 * ncols + col];
+            const T x1 = X[(i64)j1 * ncols + col];
+            const T x2 = X[(i64)j2 * ncols + col];
+            const T x3 = X[(i64)j3 * ncols + col];
+            acc0 += Cr[l] * x0;
+            acc1 += Cr[l + 1] * x1;
+            acc0 += Cr[l + 2] * x2;
+            acc1 += Cr[l + 3] * x3;
+        }
+        for (; l < L; ++l) acc0 += Cr[l] * X[(i64)Jr[l] * ncols + col];
+        Y[row * ncols + col] = acc0 + acc1;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// K4: alpha-beta part.  Workgroup = destination alpha row Ia.  For each alpha connection
+// (its nsxA single excitations Ja = Ia - p + q, then Ia itself) the source row x~[Ja, :] is
+// staged in LDS (in chunks if it does not fit) together with the integral row
+//   W[rs] = s_a (pq|rs)            (single)   or   W[rs] = sum_{p in Ia} (pp|rs)   (diagonal)
+// and every lane Ib accumulates  sum_{singles Ib<-Jb} s_b W[rs] x~[Ja,Jb] + (sum_{r in Ib} W[rr]) x~[Ja,Ib].
+// ---------------------------------------------------------------------------------------
+template <typename T, int NVS, int EPT>
+__global__ __launch_bounds__(1024) void k_ab(const T *__restrict__ xt, T *__restrict__ s1, int nvst,
+                                             int v0, const u32 *__restrict__ sxA,
+                                             const u32 *__restrict__ sxB,
+                                             const u64 *__restrict__ strsA,
+                                             const u64 *__restrict__ strsB, const T *__restrict__ eri,
+                                             int n, i64 NA, i64 NB, int nsxA, int nsxB, int ept,
+                                             i64 chunk, int nchunks, int nsplit, i64 per_split) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *W = reinterpret_cast<T *>(smem_raw);
+    const int n2 = n * n;
+    T *xrow = W + ((n2 + 15) & ~15);
+    const int tid = threadIdx.x, bd = blockDim.x;
+    // blocks are dispatched round-robin over the 8 XCDs: give each XCD a contiguous range of
+    // destination rows, and put the nsplit beta-slices of one row on the same XCD back to back
+    // so the staged source rows are served from that XCD's L2
+    const i64 per = (NA + 7) / 8;
+    const i64 jx = blockIdx.x >> 3;
+    const i64 Ia = (blockIdx.x & 7) * per + jx / nsplit;
+    const i64 ib0 = (jx % nsplit) * per_split;
+    if (Ia >= NA) return;
+    const i64 ib1 = min(NB, ib0 + per_split);
+    const u64 astr = strsA[Ia];
+
+    T acc[EPT][NVS];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e)
+#pragma unroll
+        for (int v = 0; v < NVS; ++v) acc[e][v] = 0;
+
+    for (int ca = 0; ca <= nsxA; ++ca) {
+        i64 Ja;
+        __syncthreads();
+        if (ca < nsxA) {
+            const u32 d = sxA[(i64)ca * NA + Ia];
+            Ja = sx_J(d);
+            const T sg = sx_neg(d) ? T(-1) : T(1);
+            const T *er = eri + (i64)(sx_r(d) * n + sx_s(d)) * n2;
+            for (int t = tid; t < n2; t += bd) W[t] = sg * er[t];
+        } else {
+            Ja = Ia;
+            for (int t = tid; t < n2; t += bd) {
+                T v = 0;
+                u64 m = astr;
+                while (m) {
+                    const int p = __builtin_ctzll(m);
+                    m &= m - 1;
+                    v += eri[(i64)(p * n + p) * n2 + t];
+                }
+                W[t] = v;
+            }
+        }
+        for (int c = 0; c < nchunks; ++c) {
+            const i64 base = (i64)c * chunk;
+            const i64 len = min(chunk, NB - base);
+            if (c > 0) __syncthreads();
+            const T *srow = xt + (Ja * NB + base) * nvst + v0;
+            if (NVS == 1 && nvst == 1) {
+                for (i64 t = tid; t < len; t += bd) xrow[t] = srow[t];
+            } else {
+                for (i64 t = tid; t < len * NVS; t += bd) {
+                    const i64 el = t / NVS;
+                    xrow[t] = srow[el * nvst + (t - el * NVS)];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < EPT; ++e) {
+                const i64 Ib = ib0 + tid + (i64)e * bd;
+                if (e >= ept || Ib >= ib1) continue;
+                T a[NVS];
+#pragma unroll
+                for (int v = 0; v < NVS; ++v) a[v] = 0;
+                for (int cb = 0; cb < nsxB; ++cb) {
+                    const u32 d = sxB[(i64)cb * NB + Ib];
+                    const i64 off = (i64)sx_J(d) - base;
+                    if (nchunks > 1 && (off < 0 || off >= len)) continue;
+                    T w = W[sx_r(d) * n + sx_s(d)];
+                    if (sx_neg(d)) w = -w;
+#pragma unroll
+                    for (int v = 0; v < NVS; ++v) a[v] += w * xrow[off * NVS + v];
+                }
+                const i64 offd = Ib - base;
+                if (offd >= 0 && offd < len) {
+                    T wd = 0;
+                    u64 m = strsB[Ib];
+                    while (m) {
+                        const int r = __builtin_ctzll(m);
+                        m &= m - 1;
+                        wd += W[r * n + r];
+                    }
+#pragma unroll
+                    for (int v = 0; v < NVS; ++v) a[v] += wd * xrow[offd * NVS + v];
+                }
+#pragma unroll
+                for (int v = 0; v < NVS; ++v) acc[e][v] += a[v];
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+        const i64 Ib = ib0 + tid + (i64)e * bd;
+        if (e >= ept || Ib >= ib1) continue;
+#pragma unroll
+        for (int v = 0; v < NVS; ++v) s1[(Ia * NB + Ib) * nvst + v0 + v] += acc[e][v];
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// K5: sigma = E (s1 + s2T^T + E_core x~), FP64 output vectors
+// ---------------------------------------------------------------------------------------
+template <typename T, int NV>
+__global__ __launch_bounds__(256) void k_finalize(const T *__restrict__ s1, const T *__restrict__ s2T,
+                                                  const T *__restrict__ xt, double ecore, DstPack dst,
+                                                  const u64 *__restrict__ strsB,
+                                                  const u64 *__restrict__ M, i64 NA, i64 NB) {
+    __shared__ T tile[32][33];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const i64 b0 = (i64)blockIdx.x * 32, a0 = (i64)blockIdx.y * 32;
+#pragma unroll 1
+    for (int v = 0; v < NV; ++v) {
+        for (int i = 0; i < 4; ++i) {
+            const i64 b = b0 + ty + 8 * i, a = a0 + tx;
+            if (a < NA && b < NB) tile[ty + 8 * i][tx] = s2T[(b * NA + a) * NV + v];
+        }
+        __syncthreads();
+        for (int i = 0; i < 4; ++i) {
+            const i64 a = a0 + ty + 8 * i, b = b0 + tx;
+            if (a < NA && b < NB) {
+                const i64 k = (a * NB + b) * NV + v;
+                double val = (double)s1[k] + (double)tile[tx][ty + 8 * i] + ecore * (double)xt[k];
+                if (parity64(strsB[b] & M[a])) val = -val;
+                dst.p[v][a * NB + b] = val;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// diag[i] += H_ii  (accumulates, as get_diags_elements does: src/include/op/hvec.hpp:498-554)
+__global__ __launch_bounds__(256) void k_diag(double *__restrict__ diags, const double *__restrict__ gdA,
+                                              const double *__restrict__ gdB,
+                                              const double *__restrict__ jab,
+                                              const u64 *__restrict__ strsB, int n, i64 NA, i64 NB,
+                                              double ecore) {
+    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NA * NB) return;
+    const i64 a = i / NB, b = i - a * NB;
+    double v = gdA[a] + gdB[b] + ecore;
+    u64 m = strsB[b];
+    const double *ja = jab + a * n;
+    while (m) {
+        const int r = __builtin_ctzll(m);
+        m &= m - 1;
+        v += ja[r];
+    }
+    diags[i] += v;
+}
+
+// ---------------------------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------------------------
+static size_t lds_budget() {
+    static size_t cached = 0;
+    if (!cached) {
+        int dev = 0, v = 0;
+        HIPOTF_CHECK(hipGetDevice(&dev));
+        HIPOTF_CHECK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        cached = (size_t)v;
+    }
+    return cached;
+}
+
+template <typename T>
+static void launch_rowspmm(const T *X, T *Y, const SameSpinTable &tab, const T *C, i64 nrows,
+                           i64 ncols, hipStream_t st) {
+    if (nrows == 0 || ncols == 0) return;
+    const i64 ntiles = (ncols + 63) / 64;
+    const i64 bpt = (nrows + 3) / 4;
+    const i64 ntiles_pad = (ntiles + 7) / 8 * 8;
+    const i64 nblocks = ntiles_pad * bpt;
+    HIPOTF_REQUIRE(nblocks < (1ll << 31), "row-SpMM grid too large");
+    hipLaunchKernelGGL(k_rowspmm<T>, dim3((unsigned)nblocks), dim3(256), 0, st, X, Y,
+                       tab.J.as<int32_t>(), C, tab.L, nrows, ncols, ntiles, bpt);
+    HIPOTF_CHECK(hipGetLastError());
+}
+
+template <typename T, int NVS, int EPT>
+static void launch_ab_ept(DevHam *H, const T *xt, T *s1, int nvst, int v0, const T *eri, int bd,
+                          int ept, i64 chunk, int nchunks, int nsplit, i64 per_split, size_t smem,
+                          hipStream_t st) {
+    DevBasis *B = H->basis;
+    const DevSpin &A = *B->pA, &Bs = *B->pB;
+    auto kern = k_ab<T, NVS, EPT>;
+    HIPOTF_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)smem));
+    const i64 per = (A.N + 7) / 8;
+    const i64 nblocks = per * nsplit * 8;
+    HIPOTF_REQUIRE(nblocks < (1ll << 31), "alpha-beta grid too large");
+    hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(bd), smem, st, xt, s1, nvst, v0,
+                       A.sx.as<u32>(), Bs.sx.as<u32>(), A.strs.as<u64>(), Bs.strs.as<u64>(), eri,
+                       H->n, A.N, Bs.N, A.nsx, Bs.nsx, ept, chunk, nchunks, nsplit, per_split);
+    HIPOTF_CHECK(hipGetLastError());
+}
+
+// accumulator budget per lane: EPT_MAX * NVS values of T within ~64 VGPRs (no spills at
+// 1024 threads / 128 VGPRs); a destination row wider than bd * EPT_MAX is split over blocks
+template <typename T, int NVS>
+constexpr int ab_eptmax() { return (sizeof(T) == 8 && NVS >= 4) ? 4 : 8; }
+
+template <typename T, int NVS>
+static void launch_ab_nvs(DevHam *H, const T *xt, T *s1, int nvst, int v0, const T *eri,
+                          hipStream_t st) {
+    DevBasis *B = H->basis;
+    const i64 NB = B->pB->N;
+    const int n2 = H->n * H->n;
+    const size_t wbytes = (size_t)((n2 + 15) & ~15) * sizeof(T);
+    const size_t budget = std::min<size_t>(lds_budget(), 160 * 1024) - 1024;
+    HIPOTF_REQUIRE(budget > wbytes + 64 * NVS * sizeof(T), "LDS too small for the alpha-beta kernel");
+    i64 chunk = (i64)((budget - wbytes) / (NVS * sizeof(T)));
+    if (chunk >= NB) chunk = NB;
+    const int nchunks = (int)((NB + chunk - 1) / chunk);
+    const size_t smem = wbytes + (size_t)chunk * NVS * sizeof(T);
+    const int bd = NB >= 4096 ? 1024 : (NB >= 1024 ? 512 : 256);
+    constexpr int EM = ab_eptmax<T, NVS>();
+    const int nsplit = (int)((NB + (i64)bd * EM - 1) / ((i64)bd * EM));
+    const i64 per_split = (NB + nsplit - 1) / nsplit;
+    const int ept = (int)((per_split + bd - 1) / bd);
+    if (ept <= 4) launch_ab_ept<T, NVS, 4>(H, xt, s1, nvst, v0, eri, bd, ept, chunk, nchunks, nsplit, per_split, smem, st);
+    else launch_ab_ept<T, NVS, EM>(H, xt, s1, nvst, v0, eri, bd, ept, chunk, nchunks, nsplit, per_split, smem, st);
+}
+
+// processes the nv interleaved vectors in sub-blocks of NVS so that a staged source row fits
+// LDS and the per-lane accumulators stay within ~64 VGPRs
+template <typename T>
+static void launch_ab(DevHam *H, const T *xt, T *s1, int nv, const T *eri, hipStream_t st) {
+    DevBasis *B = H->basis;
+    const i64 NB = B->pB->N;
+    if (B->pA->N == 0 || NB == 0) return;
+    const size_t budget = std::min<size_t>(lds_budget(), 160 * 1024) - 4096;
+    auto fits = [&](int s) { return (size_t)NB * s * sizeof(T) <= budget; };
+    int nvs = 1;
+    if (nv >= 4 && fits(4)) nvs = 4;
+    else if (nv >= 2 && fits(2)) nvs = 2;
+    for (int v0 = 0; v0 < nv;) {
+        const int m = std::min(nvs, nv - v0);
+        if (m >= 4) { launch_ab_nvs<T, 4>(H, xt, s1, nv, v0, eri, st); v0 += 4; }
+        else if (m >= 2) { launch_ab_nvs<T, 2>(H, xt, s1, nv, v0, eri, st); v0 += 2; }
+        else { launch_ab_nvs<T, 1>(H, xt, s1, nv, v0, eri, st); v0 += 1; }
+    }
+}
+
+template <typename T, int NV>
+static void sigma_group(DevHam *H, const SrcPack &src, const DstPack &dst, hipStream_t st) {
+    DevBasis *B = H->basis;
+    const DevSpin &A = *B->pA, &Bs = *B->pB;
+    const i64 NA = A.N, NB = Bs.N, dim = NA * NB;
+    if (dim == 0) return;
+    const size_t bytes = (size_t)dim * NV * sizeof(T);
+    B->xt.ensure(bytes);
+    B->xtT.ensure(bytes);
+    B->s1.ensure(bytes);
+    B->s2T.ensure(bytes);
+    T *xt = B->xt.as<T>(), *xtT = B->xtT.as<T>(), *s1 = B->s1.as<T>(), *s2T = B->s2T.as<T>();
+    dim3 g2((unsigned)((NB + 31) / 32), (unsigned)((NA + 31) / 32));
+    HIPOTF_REQUIRE(g2.y < 65536, "too many alpha strings for the tiled kernels");
+    hipLaunchKernelGGL((k_gauge_in<T, NV>), g2, dim3(256), 0, st, src, xt, xtT, Bs.strs.as<u64>(),
+                       B->gaugeM.as<u64>(), NA, NB);
+    HIPOTF_CHECK(hipGetLastError());
+    const bool f32 = sizeof(T) == 4;
+    const T *CA = f32 ? (const T *)H->ptA->c32.p : (const T *)H->ptA->c64.p;
+    const T *CB = f32 ? (const T *)H->ptB->c32.p : (const T *)H->ptB->c64.p;
+    launch_rowspmm<T>(xt, s1, *H->ptA, CA, NA, NB * NV, st);
+    launch_rowspmm<T>(xtT, s2T, *H->ptB, CB, NB, NA * NV, st);
+    const T *eri = f32 ? (const T *)H->eri32.p : (const T *)H->eri64.p;
+    launch_ab<T>(H, xt, s1, NV, eri, st);
+    hipLaunchKernelGGL((k_finalize<T, NV>), g2, dim3(256), 0, st, s1, s2T, xt, H->ecore, dst,
+                       Bs.strs.as<u64>(), B->gaugeM.as<u64>(), NA, NB);
+    HIPOTF_CHECK(hipGetLastError());
+}
+
+template <typename T>
+static void sigma_t(DevHam *H, int nv, const double *const *src, double *const *dst, hipStream_t st) {
+    for (int g0 = 0; g0 < nv; g0 += MAXNV) {
+        const int m = std::min(MAXNV, nv - g0);
+        SrcPack sp{};
+        DstPack dp{};
+        for (int i = 0; i < m; ++i) {
+            HIPOTF_REQUIRE(src[g0 + i] && dst[g0 + i], "NULL vector pointer");
+            sp.p[i] = src[g0 + i];
+            dp.p[i] = dst[g0 + i];
+        }
+        switch (m) {
+            case 1: sigma_group<T, 1>(H, sp, dp, st); break;
+            case 2: sigma_group<T, 2>(H, sp, dp, st); break;
+            case 3: sigma_group<T, 3>(H, sp, dp, st); break;
+            case 4: sigma_group<T, 4>(H, sp, dp, st); break;
+            case 5: sigma_group<T, 5>(H, sp, dp, st); break;
+            case 6: sigma_group<T, 6>(H, sp, dp, st); break;
+            case 7: sigma_group<T, 7>(H, sp, dp, st); break;
+            default: sigma_group<T, 8>(H, sp, dp, st); break;
+        }
+    }
+}
+
+void sigma(DevHam *H, int nv, const double *const *src, double *const *dst, int prec, hipStream_t st) {
+    HIPOTF_REQUIRE(H != nullptr, "Hamiltonian handle is NULL");
+    HIPOTF_REQUIRE(nv >= 1, "need at least one vector");
+    if (prec == 0) sigma_t<double>(H, nv, src, dst, st);
+    else sigma_t<float>(H, nv, src, dst, st);
+}
+
+void diag_elements(DevHam *H, double *diags, hipStream_t st) {
+    HIPOTF_REQUIRE(H && diags, "NULL argument");
+    DevBasis *B = H->basis;
+    const i64 NA = B->pA->N, NB = B->pB->N, dim = NA * NB;
+    if (dim == 0) return;
+    hipLaunchKernelGGL(k_diag, dim3((unsigned)((dim + 255) / 256)), dim3(256), 0, st, diags,
+                       H->ptA->gdiag.as<double>(), H->ptB->gdiag.as<double>(), H->jab.as<double>(),
+                       B->pB->strs.as<u64>(), H->n, NA, NB, H->ecore);
+    HIPOTF_CHECK(hipGetLastError());
+}
+
+}  // namespace hipotf

@@ -1,0 +1,132 @@
+/*
+ * libhipotf -- MI355X (gfx950) on-the-fly FCI / VQE operator library.
+ *
+ * C ABI: plain pointers and sizes only.  Every vector argument of the device entry points is a
+ * DEVICE pointer to float64[dim] in the reference determinant order (|a,b> at ia*Nb + ib,
+ * colex-ordered strings, interleaved Jordan-Wigner signs).  Small host-side arrays (Ritz
+ * coefficients, heff, thetas, gradients of the fused VQE call) are host pointers, as in the
+ * Julia callers.  Nothing here exits the process: failures return NULL / NaN / nonzero and
+ * leave a message in hipotf_last_error().  All work is issued on the default stream; entry
+ * points that return a value synchronise.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   libcuotf.so      src/extern/cuotf.cu:7-162          (callers: jl/cuda/cunetwork.jl)
+ *   libbasis.so      src/include/basis/declare.hpp:4-81  (callers: jl/network.jl:1-116)
+ *   libotf_native.so src/include/op/declare.hpp:13-38    (build/destroy of the network handle)
+ *   libdiag.so       src/include/diag/declare.hpp:4-75  (callers: jl/davidson.jl)
+ */
+#ifndef HIPOTF_H
+#define HIPOTF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- errors (no reference counterpart: the reference exits via CUDA_CHECK,
+ *      src/include/cuda/common.cuh:42-51) ---------------------------------------------------- */
+const char *hipotf_last_error(void);
+void hipotf_clear_error(void);
+
+/* ---- host basis: replaces libbasis.so (src/include/basis/declare.hpp:4-81) ---------------- */
+/* create_basis_manager: basis/declare.hpp:24-33 -> basis.hpp:118-296.  Only the single-block
+ * case (orbsym all 0) is accepted; total_sym != 0 yields an empty space as the reference does. */
+void *create_basis_manager(int64_t norb, int64_t na, int64_t nb, int64_t total_sym,
+                           const int64_t *orbsym, int64_t num_irreps);
+int64_t get_subspace_dim(void *basis);          /* basis/declare.hpp:6-9 */
+int64_t get_num_symmetry_blocks(void *basis);   /* basis/declare.hpp:11-14 */
+void destroy_basis_manager(void *basis);        /* basis/declare.hpp:16-21 */
+/* set_det_coeff_f64: basis/declare.hpp:35-45 -> basis.hpp:298-330 (vec is a HOST array) */
+void set_det_coeff_f64(void *basis, uint32_t astr, uint32_t bstr, double coeff, double *vec);
+
+/* ---- host operator network: replaces build/destroy_network_otf_f64 of libotf_native.so
+ *      (src/include/op/declare.hpp:15-38, ham/otf.hpp:223-349).  Same flat SVD-group arrays
+ *      as jl/network.jl:313-339 passes. ---------------------------------------------------- */
+void *build_network_otf_f64(const int64_t *orbsym, int64_t norb, int64_t ngs, const uint32_t *axs,
+                            const uint32_t *bxs, const int64_t *ranks, const int64_t *num_zas,
+                            const int64_t *num_zbs, const uint32_t *flat_zas,
+                            const uint32_t *flat_zbs, const double *flat_wa,
+                            const double *flat_wb);
+void destroy_network_otf_f64(void *net);
+
+/* ---- device handles and operators: replaces libcuotf.so (src/extern/cuotf.cu:7-162) ------- */
+void sync_device_cuda(void);                              /* cuotf.cu:9-12 */
+void *build_basisdev_f64(void *host_basis);               /* cuotf.cu:25-30 */
+void destroy_basisdev_f64(void *basis);                   /* cuotf.cu:14-23 */
+/* build_networkdev_f64 (cuotf.cu:43-48): decodes a host network from build_network_otf_f64.
+ * A network with a diagonal group is read as a two-body Hamiltonian (integrals recovered by
+ * probing matrix elements); otherwise every group must be +-(T - T^dagger) of one excitation. */
+void *build_networkdev_f64(void *host_net);
+void destroy_networkdev_f64(void *net);                   /* cuotf.cu:32-41 */
+/* hvec_cuda (cuotf.cu:50-60): dst = H src (dst overwritten, hvec.cuh:439) */
+void hvec_cuda(void *basis, void *net, const double *src, double *dst);
+/* get_diags_elements_cuda (cuotf.cu:62-71): diags[i] += H_ii (accumulates) */
+void get_diags_elements_cuda(void *basis, void *net, double *diags);
+/* expm_cuda (cuotf.cu:73-83): vec <- exp(theta G_idx) vec, idx 0-based pool index */
+void expm_cuda(void *basis, void *net, int64_t idx, double theta, double *vec);
+/* grad_cuda (cuotf.cu:85-96): sum over pairs of grad_update (core/math.hpp:148-162) */
+double grad_cuda(void *basis, void *net, int64_t idx, double theta, const double *lp,
+                 const double *rp);
+/* backgrad_cuda (cuotf.cu:98-109): lp <- R(-theta) lp; g; rp <- R(-theta) rp (math.hpp:187-215) */
+double backgrad_cuda(void *basis, void *net, int64_t idx, double theta, double *lp, double *rp);
+/* batchgrad_cuda (cuotf.cu:111-122): grads[g] = grad(g, thetas[g], lp, rp); device arrays */
+void batchgrad_cuda(void *basis, void *net, const double *thetas, const double *lp,
+                    const double *rp, double *grads);
+void expm_cuda_2d(void *basis, void *net, int64_t idx, double theta, double *vec); /* cuotf.cu:124-133 */
+double grad_cuda_2d(void *basis, void *net, int64_t idx, double theta, const double *lp,
+                    const double *rp);                                            /* cuotf.cu:135-146 */
+double backgrad_cuda_2d(void *basis, void *net, int64_t idx, double theta, double *lp,
+                        double *rp);                                              /* cuotf.cu:148-159 */
+
+/* ---- constructors added beside the network path (no reference counterpart: the reference
+ *      only accepts SVD networks, jl/cuda/cunetwork.jl:38-53) ------------------------------- */
+/* Hamiltonian from the same integral buffers generate_hamiltonian_real_64_128_f64 takes
+ * (src/include/ham/declare.hpp:7-27): one_body_mo[p + q n] = h_pq,
+ * two_body_mo[p + q n + r n^2 + s n^3] = (ps|qr) (column-major), energy_nuc = E_core. */
+void *hipotf_build_ham_f64(int64_t norb, const double *one_body_mo, const double *two_body_mo,
+                           double energy_nuc);
+/* Generator pool from spin-orbital lists in the format of generate_ci_spin_orbitals
+ * (jl/ansatz.jl:67-167): op g uses orbs[offsets[g] .. offsets[g+1]), creators first, then
+ * annihilators (spin-orbital 2p = alpha_p, 2p+1 = beta_p); G_g = T_g - T_g^dagger. */
+void *hipotf_build_pool_f64(int64_t norb, int64_t nops, const int32_t *offsets,
+                            const int32_t *orbs);
+int64_t hipotf_num_generators(void *net);
+/* builds the basis-dependent device tables of an operator now instead of on first use */
+int hipotf_prepare(void *basis, void *net);
+/* block sigma: dst[v] = H src[v], v < nvec (host arrays of device pointers).
+ * precision 0: float64 arithmetic (as hvec_cuda); 1: float32 coefficients and integrals with
+ * float64 input/output vectors (the mixed-precision Davidson bulk). */
+int hipotf_hvec_block(void *basis, void *net, int nvec, const double *const *src,
+                      double *const *dst, int precision);
+/* fused VQE objective of jl/vqe.jl:1-21 (energy_objective): forward expm sweep over
+ * idxs/thetas (host arrays), lam = H psi, energy, variance, reverse backgrad sweep.
+ * psi: device, in = reference state, out = reference state after the uncomputing sweep;
+ * lam: device workspace; grads: host [nparams] = 2 Re(backgrad)/|psi|^2;
+ * out3: host {energy, max(0, |H psi|^2/|psi|^2 - E^2), |psi|^2}.  Returns 0 on success. */
+int hipotf_vqe_energy_grad(void *basis, void *ham, void *pool, int64_t nparams, const int64_t *idxs,
+                           const double *thetas, double *psi, double *lam, double *grads,
+                           double *out3);
+
+/* ---- Davidson inner loop on the device: replaces libdiag.so (src/include/diag/declare.hpp).
+ *      V / AV are HOST arrays of DEVICE vector pointers (<= 64); coeffs, out_col and heff are
+ *      host arrays. ---------------------------------------------------------------------- */
+double build_ritz_and_residual_f64(int64_t N, int dim, const double *coeffs, double min_e,
+                                   const double **V, const double **AV, double *axt); /* :7-15 */
+void build_ritz_vector_f64(int64_t N, int dim, const double *coeffs, const double **V,
+                           double *xt);                                                  /* :17-24 */
+void apply_preconditioner_inplace_f64(int64_t N, double *axt, const double *diags, double min_e,
+                                      double shift);                                     /* :26-33 */
+double mgs_orthogonalize_f64(int64_t N, int dim, const double **V, double *vt);          /* :35-41 */
+double inplace_normalize_f64(int64_t N, double *vec);                                    /* :43-48 */
+double fast_dot_f64(int64_t N, const double *v1, const double *v2);                      /* :50-56 */
+void compute_heff_col_f64(int64_t N, int dim, const double **V, const double *AV_dim,
+                          double *out_col);                                              /* :58-65 */
+void rebuild_heff_f64(int64_t N, int dim, const double **V, const double **AV, double *heff,
+                      int ldh);                                                          /* :67-75 */
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HIPOTF_H */

@@ -1,0 +1,381 @@
+"""ORACLE (test infrastructure only) -- numpy restatement of the reference's host-side glue.
+
+Nothing in the product path imports this module; only tests/, the golden-fixture script,
+``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg do, and only as the checker.
+
+Restated pieces (each function cites what it follows):
+
+* :func:`jw_hamiltonian_terms` -- integrals -> interleaved Jordan-Wigner Pauli terms, the
+  real-integral engine of src/include/ham/ham.hpp:635-1069, split into AABB halves
+  (ham.hpp:519-551).
+* :func:`group_bounds` -- jl/tools.jl:25-49 ``get_bounds_0based``.
+* :func:`compress_by_svd` / :func:`compress_pool_by_svd` -- jl/network.jl:158-246 and
+  jl/network.jl:248-301 (numpy SVD instead of OpenBLAS: the factors are not unique, but
+  every group element f_g = sum_r Pa_r Pb_r is, which is all the kernels consume).
+* :func:`fermion_operator` / :func:`ucc_generator` -- jl/binqubitaabb.jl:313-428 and
+  jl/ansatz.jl:230-276 (``aabb_a`` ladder operators, ``multiply``, ``t - t'``).
+* :func:`ci_spin_orbitals` -- jl/ansatz.jl:67-167 (parameter order; identity popped).
+* :func:`hf_strings` -- jl/network.jl:81-116.
+
+Pauli convention (jl/tools.jl:101-103): a term c X^x Z^z maps |s> to c (-1)^popc(z&s) |s^x>;
+bit 2p is alpha_p and bit 2p+1 is beta_p.
+"""
+from __future__ import annotations
+
+from itertools import combinations
+
+import numpy as np
+
+EPS2 = 1e-12  # eps2 of the reference's Pauli algebra (tol of multiply / linearcombine)
+
+
+# ----------------------------------------------------------------------------------------
+# bit helpers (src/include/core/bit.hpp:59-133)
+# ----------------------------------------------------------------------------------------
+def zip_even(v: int) -> int:
+    """Gather the even bits of v (alpha half)."""
+    out, i = 0, 0
+    while v:
+        if v & 1:
+            out |= 1 << i
+        v >>= 2
+        i += 1
+    return out
+
+
+def zip_odd(v: int) -> int:
+    return zip_even(v >> 1)
+
+
+def unzip_even(a: int) -> int:
+    out, i = 0, 0
+    while a:
+        if a & 1:
+            out |= 1 << (2 * i)
+        a >>= 1
+        i += 1
+    return out
+
+
+def unzip_odd(b: int) -> int:
+    return unzip_even(b) << 1
+
+
+def popc(v: int) -> int:
+    return bin(v).count("1")
+
+
+# ----------------------------------------------------------------------------------------
+# integrals -> Pauli terms (src/include/ham/ham.hpp real engine)
+# ----------------------------------------------------------------------------------------
+def _add(d, x, z, c):
+    key = (x, z)
+    d[key] = d.get(key, 0.0) + c
+
+
+def _insert_1body_real(d, p, q, coeff):
+    """ham.hpp:635-669 (one representative of (p,q),(q,p), coeff already doubled)."""
+    if p == q:
+        cc = 0.5 * coeff
+        _add(d, 0, 0, cc)
+        _add(d, 0, 1 << p, -cc)
+        return
+    x1, x2 = 1 << p, 1 << q
+    z11, z12 = x1 - 1, (x1 << 1) - 1
+    z21, z22 = x2 - 1, (x2 << 1) - 1
+    cc = 0.25 * coeff
+    c1 = -cc if popc(z11 & x2) & 1 else cc
+    c2 = -cc if popc(z12 & x2) & 1 else cc
+    x12 = x1 ^ x2
+    _add(d, x12, z11 ^ z22, -c1)
+    _add(d, x12, z12 ^ z21, c2)
+
+
+def _two_body_parities(p, q, r, s):
+    x1, x2, x3, x4 = 1 << p, 1 << q, 1 << r, 1 << s
+    z = [(x1 - 1, (x1 << 1) - 1), (x2 - 1, (x2 << 1) - 1),
+         (x3 - 1, (x3 << 1) - 1), (x4 - 1, (x4 << 1) - 1)]
+    x34, x234 = x3 ^ x4, x2 ^ x3 ^ x4
+    p11 = popc(z[0][0] & x234) & 1
+    p12 = popc(z[0][1] & x234) & 1
+    p21 = popc(z[1][0] & x34) & 1
+    p22 = popc(z[1][1] & x34) & 1
+    p31 = popc(z[2][0] & x4) & 1
+    p32 = popc(z[2][1] & x4) & 1
+    return x1 ^ x2 ^ x3 ^ x4, z, (p11 ^ p21, p11 ^ p22, p12 ^ p21, p12 ^ p22), (p31, p32)
+
+
+def _insert_2body_real(d, p, q, r, s, coeff):
+    """ham.hpp:672-725: the 8 real Pauli strings of 1/2 a+_p a+_q a_r a_s + h.c. partners."""
+    x, z, (pa, pb, pc, pd), (p31, p32) = _two_body_parities(p, q, r, s)
+    cc = 0.0625 * coeff
+
+    def sg(par):
+        return -cc if par else cc
+
+    c11, c12 = sg(pa ^ p31), sg(pa ^ p32)
+    c21, c22 = sg(pb ^ p31), sg(pb ^ p32)
+    c31, c32 = sg(pc ^ p31), sg(pc ^ p32)
+    c41, c42 = sg(pd ^ p31), sg(pd ^ p32)
+    (z11, z12), (z21, z22), (z31, z32), (z41, z42) = z
+    _add(d, x, z11 ^ z21 ^ z31 ^ z41, c11)
+    _add(d, x, z11 ^ z21 ^ z32 ^ z42, c12)
+    _add(d, x, z11 ^ z22 ^ z31 ^ z42, -c21)
+    _add(d, x, z11 ^ z22 ^ z32 ^ z41, -c22)
+    _add(d, x, z12 ^ z21 ^ z31 ^ z42, -c31)
+    _add(d, x, z12 ^ z21 ^ z32 ^ z41, -c32)
+    _add(d, x, z12 ^ z22 ^ z31 ^ z41, c41)
+    _add(d, x, z12 ^ z22 ^ z32 ^ z42, c42)
+
+
+def _insert_2body_general(d, p, q, r, s, coeff):
+    """ham.hpp:112-180: the full 16-term expansion."""
+    x, z, (pa, pb, pc, pd), (p31, p32) = _two_body_parities(p, q, r, s)
+    cc = 0.0625 * coeff
+
+    def sg(par):
+        return -cc if par else cc
+
+    (z11, z12), (z21, z22), (z31, z32), (z41, z42) = z
+    rows = ((z11, z21, pa), (z11, z22, pb), (z12, z21, pc), (z12, z22, pd))
+    for za, zb, par in rows:
+        c1, c2 = sg(par ^ p31), sg(par ^ p32)
+        _add(d, x, za ^ zb ^ z31 ^ z41, c1)
+        _add(d, x, za ^ zb ^ z31 ^ z42, -c1)
+        _add(d, x, za ^ zb ^ z32 ^ z41, -c2)
+        _add(d, x, za ^ zb ^ z32 ^ z42, c2)
+
+
+def _spin_blocks(p, q, r, s):
+    return ((2 * p, 2 * q, 2 * r, 2 * s), (2 * p + 1, 2 * q + 1, 2 * r + 1, 2 * s + 1),
+            (2 * p, 2 * q + 1, 2 * r + 1, 2 * s), (2 * p + 1, 2 * q, 2 * r, 2 * s + 1))
+
+
+def jw_hamiltonian_terms(h1: np.ndarray, g2: np.ndarray, n: int, tol: float = 1e-12,
+                         e_nuc: float = 0.0):
+    """Pauli terms of the qubit Hamiltonian, sorted by (x, z) as ham.hpp:1037 does.
+
+    ``h1``/``g2`` are the reference's column-major flat buffers (see
+    binsim_amd.integrals.reference_layout).  Returns (axs, bxs, azs, bzs, cs) with uint32
+    alpha/beta halves (ham.hpp:519-551) -- the BinaryQubitAABB the Julia layer builds.
+    Follows stage1 (ham.hpp:841-918), stage2 (ham.hpp:920-996), stage3 (ham.hpp:462-505).
+    """
+    N1, N2, N3 = n, n * n, n * n * n
+    d: dict = {}
+    for p in range(n):
+        for q in range(p, n):
+            val = h1[p + q * N1]
+            if abs(val) > tol:
+                w = 1.0 if p == q else 2.0
+                _insert_1body_real(d, 2 * p, 2 * q, val * w)
+                _insert_1body_real(d, 2 * p + 1, 2 * q + 1, val * w)
+    for s in range(n):
+        for r in range(n):
+            for q in range(n):
+                for p in range(n):
+                    i1 = s * N3 + r * N2 + q * N1 + p
+                    i2 = r * N3 + s * N2 + p * N1 + q
+                    i3 = p * N3 + q * N2 + r * N1 + s
+                    i4 = q * N3 + p * N2 + s * N1 + r
+                    if i1 != max(i1, i2, i3, i4):
+                        continue
+                    val = g2[p + q * N1 + r * N2 + s * N3]
+                    if abs(val) <= tol:
+                        continue
+                    dup = 1 + (i1 == i2) + (i1 == i3) + (i1 == i4)
+                    cw = val * (4.0 / dup)
+                    if len({p, q, r, s}) == 4:
+                        for blk in _spin_blocks(p, q, r, s):
+                            _insert_2body_real(d, *blk, cw * 0.5)
+                    else:
+                        members = [(p, q, r, s), (q, p, s, r), (s, r, q, p), (r, s, p, q)]
+                        uniq = []
+                        for m in members:
+                            if m not in uniq:
+                                uniq.append(m)
+                        raw = (cw / len(uniq)) * 0.5
+                        for m in uniq:
+                            for blk in _spin_blocks(*m):
+                                _insert_2body_general(d, *blk, raw)
+    if abs(e_nuc) > tol:
+        _add(d, 0, 0, e_nuc)
+    terms = sorted((k for k, v in d.items() if abs(v) > tol))
+    axs = np.array([zip_even(x) for x, _ in terms], dtype=np.uint32)
+    bxs = np.array([zip_odd(x) for x, _ in terms], dtype=np.uint32)
+    azs = np.array([zip_even(z) for _, z in terms], dtype=np.uint32)
+    bzs = np.array([zip_odd(z) for _, z in terms], dtype=np.uint32)
+    cs = np.array([d[k] for k in terms], dtype=np.float64)
+    return axs, bxs, azs, bzs, cs
+
+
+# ----------------------------------------------------------------------------------------
+# grouping + SVD compression (jl/tools.jl, jl/network.jl)
+# ----------------------------------------------------------------------------------------
+def group_bounds(axs, bxs):
+    """jl/tools.jl:25-49: 0-based bounds of runs of equal interleaved x."""
+    n = len(axs)
+    if n == 0:
+        return [0, 0]
+    if n == 1:
+        return [0, 1]
+    b = [0]
+    for i in range(1, n):
+        if axs[i] != axs[i - 1] or bxs[i] != bxs[i - 1]:
+            b.append(i)
+    b.append(n)
+    return b
+
+
+class Group:
+    __slots__ = ("ax", "bx", "rank", "azs", "bzs", "wa", "wb")
+
+    def __init__(self, ax, bx, azs, bzs, wa, wb):
+        self.ax, self.bx = int(ax), int(bx)
+        self.azs = np.asarray(azs, dtype=np.uint32)
+        self.bzs = np.asarray(bzs, dtype=np.uint32)
+        self.wa = np.asarray(wa, dtype=np.float64)  # [num_za, rank]
+        self.wb = np.asarray(wb, dtype=np.float64)  # [num_zb, rank]
+        self.rank = self.wa.shape[1]
+
+
+def _svd_group(ax, bx, azs, bzs, cs, tol):
+    ua = np.unique(np.asarray(azs, dtype=np.uint32))
+    ub = np.unique(np.asarray(bzs, dtype=np.uint32))
+    ia = {int(z): i for i, z in enumerate(ua)}
+    ib = {int(z): i for i, z in enumerate(ub)}
+    M = np.zeros((len(ua), len(ub)))
+    for za, zb, c in zip(azs, bzs, cs):
+        M[ia[int(za)], ib[int(zb)]] += c
+    U, S, Vt = np.linalg.svd(M, full_matrices=False)
+    keep = [r for r in range(len(S)) if abs(S[r]) > tol]
+    wa = np.zeros((len(ua), len(keep)))
+    wb = np.zeros((len(ub), len(keep)))
+    for ri, r in enumerate(keep):
+        sq = np.sqrt(S[r])
+        wa[:, ri] = U[:, r] * sq
+        wb[:, ri] = Vt[r, :] * sq
+    wa[np.abs(wa) < 1e-12] = 0.0
+    wb[np.abs(wb) < 1e-12] = 0.0
+    return Group(ax, bx, ua, ub, wa, wb)
+
+
+def compress_by_svd(terms, tol: float = 1e-12):
+    """jl/network.jl:158-246 for a Hamiltonian (one group per distinct x)."""
+    axs, bxs, azs, bzs, cs = terms
+    gs = group_bounds(axs, bxs)
+    return [_svd_group(axs[lb], bxs[lb], azs[lb:rb], bzs[lb:rb], cs[lb:rb], tol)
+            for lb, rb in zip(gs[:-1], gs[1:])]
+
+
+def compress_pool_by_svd(pool, tol: float = 1e-12):
+    """jl/network.jl:248-301: one group per pool operator (each must have a single x)."""
+    out = []
+    for axs, bxs, azs, bzs, cs in pool:
+        assert len(group_bounds(axs, bxs)) - 1 <= 1
+        out.append(_svd_group(axs[-1], bxs[-1], azs, bzs, cs, tol))
+    return out
+
+
+def flatten_network(groups):
+    """Flat arrays in the order jl/network.jl:313-339 passes to build_network_otf_f64."""
+    ngs = len(groups)
+    axs = np.array([g.ax for g in groups], dtype=np.uint32)
+    bxs = np.array([g.bx for g in groups], dtype=np.uint32)
+    ranks = np.array([g.rank for g in groups], dtype=np.int64)
+    nza = np.array([len(g.azs) for g in groups], dtype=np.int64)
+    nzb = np.array([len(g.bzs) for g in groups], dtype=np.int64)
+    fza = np.concatenate([g.azs for g in groups]).astype(np.uint32) if ngs else np.zeros(0, np.uint32)
+    fzb = np.concatenate([g.bzs for g in groups]).astype(np.uint32) if ngs else np.zeros(0, np.uint32)
+    # Julia vec() of a column-major [num_z, rank] matrix: column r contiguous
+    fwa = np.concatenate([g.wa.T.reshape(-1) for g in groups]) if ngs else np.zeros(0)
+    fwb = np.concatenate([g.wb.T.reshape(-1) for g in groups]) if ngs else np.zeros(0)
+    return dict(ngs=ngs, axs=axs, bxs=bxs, ranks=ranks, num_zas=nza, num_zbs=nzb,
+                flat_zas=fza, flat_zbs=fzb, flat_wa=np.ascontiguousarray(fwa, dtype=np.float64),
+                flat_wb=np.ascontiguousarray(fwb, dtype=np.float64))
+
+
+# ----------------------------------------------------------------------------------------
+# Pauli algebra for generators (jl/binqubitaabb.jl)
+# ----------------------------------------------------------------------------------------
+def _reduce(terms, tol):
+    """sort! + reduce_by_xz! of binqubitaabb.jl: merge equal (x,z), drop |c|<=tol."""
+    acc: dict = {}
+    for x, z, c in terms:
+        acc[(x, z)] = acc.get((x, z), 0.0) + c
+    return [(x, z, c) for (x, z), c in sorted(acc.items()) if abs(c) > tol]
+
+
+def ladder(pos: int, creation: bool):
+    """binqubitaabb.jl:385-406 ``aabb_a``: 1/2 X_pos Z_<pos +- 1/2 X_pos Z_<=pos."""
+    x = 1 << pos
+    return [(x, x - 1, 0.5), (x, (x << 1) - 1, 0.5 if creation else -0.5)]
+
+
+def multiply(ops, tol: float = EPS2):
+    """binqubitaabb.jl:313-374: left-to-right product with phase (-1)^popc(z1 & x2)."""
+    cur = [(0, 0, 1.0)]
+    for op in ops:
+        nxt = []
+        for x1, z1, c1 in cur:
+            for x2, z2, c2 in op:
+                c3 = c1 * c2
+                if abs(c3) > tol:
+                    sgn = -1.0 if popc(z1 & x2) & 1 else 1.0
+                    nxt.append((x1 ^ x2, z1 ^ z2, c3 * sgn))
+        cur = _reduce(nxt, tol)
+        if not cur:
+            return []
+    return cur
+
+
+def adjoint(terms):
+    """binqubitaabb.jl:192-197 for real coefficients: c -> c (-1)^popc(x&z)."""
+    return [(x, z, c * (-1.0 if popc(x & z) & 1 else 1.0)) for x, z, c in terms]
+
+
+def fermion_operator(term):
+    """binqubitaabb.jl:424-428 FermionOperatorAABB: product of ladder ops, in order."""
+    return multiply([ladder(pos, bool(cre)) for pos, cre in term])
+
+
+def ucc_generator(spin_orbital):
+    """jl/ansatz.jl:230-250: G = T - T^dagger with T = a+_{vir..} a_{occ..}."""
+    mid = len(spin_orbital) // 2
+    term = [(int(i), 1) for i in spin_orbital[:mid]] + [(int(i), 0) for i in spin_orbital[mid:]]
+    t = fermion_operator(term)
+    td = adjoint(t)
+    g = _reduce([(x, z, c) for x, z, c in t] + [(x, z, -c) for x, z, c in td], EPS2)
+    axs = np.array([zip_even(x) for x, _, _ in g], dtype=np.uint32)
+    bxs = np.array([zip_odd(x) for x, _, _ in g], dtype=np.uint32)
+    azs = np.array([zip_even(z) for _, z, _ in g], dtype=np.uint32)
+    bzs = np.array([zip_odd(z) for _, z, _ in g], dtype=np.uint32)
+    cs = np.array([c for _, _, c in g], dtype=np.float64)
+    return axs, bxs, azs, bzs, cs
+
+
+def ci_spin_orbitals(norb: int, na: int, nb: int, kmax: int = 2):
+    """jl/ansatz.jl:67-167 (generalize=false, all orbsym 0): the VQE parameter order.
+
+    Loop nest: ka = 0..min(na,kmax) [alpha occ combos outer, alpha vir combos inner] x
+    kb = 0..min(nb, kmax-ka) [beta occ outer, beta vir inner]; each entry is
+    vir_a + vir_b + occ_a + occ_b spin-orbital indices (alpha 2p, beta 2p+1); the first
+    (identity) entry is popped.
+    """
+    occ_a, vir_a = range(0, na), range(na, norb)
+    occ_b, vir_b = range(0, nb), range(nb, norb)
+    out = []
+    for ka in range(0, min(na, kmax) + 1):
+        for oa in combinations(occ_a, ka):
+            for va in combinations(vir_a, ka):
+                for kb in range(0, min(nb, kmax - ka) + 1):
+                    for ob in combinations(occ_b, kb):
+                        for vb in combinations(vir_b, kb):
+                            out.append([2 * v for v in va] + [2 * v + 1 for v in vb]
+                                       + [2 * o for o in oa] + [2 * o + 1 for o in ob])
+    out.pop(0)
+    return out
+
+
+def hf_strings(na: int, nb: int):
+    """jl/network.jl:81-116: the lowest na (nb) orbitals occupied."""
+    return (1 << na) - 1, (1 << nb) - 1
