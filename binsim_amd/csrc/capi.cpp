@@ -23,6 +23,7 @@ double normalize(i64 N, double *vec, hipStream_t st);
 double dot(i64 N, const double *a, const double *b, hipStream_t st);
 void dot_dev(i64 N, const double *a, const double *b, double *dres, hipStream_t st);
 void multidot(i64 N, int dim, const double *const *V, const double *a, double *out_host, hipStream_t st);
+void validate_pool(int n, i64 nops, const int32_t *offsets, const int32_t *orbs);
 
 // ------------------------------------------------------------------------------------------
 // host network: the flat SVD-group arrays of build_network_otf (src/include/ham/otf.hpp:223-349)
@@ -382,6 +383,7 @@ void *hipotf_build_ham_f64(int64_t norb, const double *one_body_mo, const double
 void *hipotf_build_pool_f64(int64_t norb, int64_t nops, const int32_t *offsets, const int32_t *orbs) {
     API_TRY
     HIPOTF_REQUIRE(norb >= 1 && norb <= 32 && nops >= 0, "bad pool sizes");
+    validate_pool((int)norb, nops, offsets, orbs);
     auto D = std::make_unique<DevNet>();
     D->kind = 2;
     D->n = (int)norb;

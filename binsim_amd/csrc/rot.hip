@@ -172,6 +172,35 @@ static int pattern_id(DevPool &P, const SpinSpace &S, int spin, u64 cre, u64 ann
     return (int)P.lists.size() - 1;
 }
 
+// structural checks of a generator list (no basis needed): creators first, <= 8 ladder
+// operators, no repeated spin-orbital, N_alpha and N_beta conserved, no number operators
+void validate_pool(int n, i64 nops, const int32_t *offsets, const int32_t *orbs) {
+    HIPOTF_REQUIRE(nops >= 0 && (nops == 0 || (offsets && orbs)), "bad pool arrays");
+    HIPOTF_REQUIRE(nops == 0 || offsets[0] == 0, "offsets[0] must be 0");
+    for (i64 g = 0; g < nops; ++g) {
+        const int32_t o0 = offsets[g], o1 = offsets[g + 1];
+        const int len = o1 - o0;
+        const std::string tag = "generator " + std::to_string(g) + ": ";
+        HIPOTF_REQUIRE(len >= 2 && len <= 8 && len % 2 == 0,
+                       tag + "need 2..8 spin-orbitals, creators first");
+        u64 m[2][2] = {{0, 0}, {0, 0}};  // [spin][cre]
+        for (int i = 0; i < len; ++i) {
+            const int pos = orbs[o0 + i];
+            HIPOTF_REQUIRE(pos >= 0 && pos < 2 * n, tag + "spin-orbital out of range");
+            const u64 bit = 1ull << (pos >> 1);
+            u64 &mm = m[pos & 1][i < len / 2 ? 1 : 0];
+            HIPOTF_REQUIRE((mm & bit) == 0, tag + "repeats a spin-orbital (T = 0)");
+            mm |= bit;
+        }
+        for (int sp = 0; sp < 2; ++sp) {
+            HIPOTF_REQUIRE((m[sp][0] & m[sp][1]) == 0,
+                           tag + "number-operator (diagonal) generators are not supported");
+            HIPOTF_REQUIRE(__builtin_popcountll(m[sp][0]) == __builtin_popcountll(m[sp][1]),
+                           tag + "does not conserve N_alpha and N_beta");
+        }
+    }
+}
+
 DevPool *make_dev_pool(DevBasis *b, i64 nops, const int32_t *offsets, const int32_t *orbs) {
     HIPOTF_REQUIRE(b != nullptr, "basis handle is NULL");
     HIPOTF_REQUIRE(nops >= 0 && (nops == 0 || (offsets && orbs)), "bad pool arrays");

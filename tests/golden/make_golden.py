@@ -1,0 +1,184 @@
+"""Writes the golden fixtures of tests/golden/ from the reference's own compiled CPU libraries.
+
+Runs only where the reference sources are present (oracle/Makefile builds oracle/_ref from
+them).  The Julia glue the reference runs around its libraries (JW grouping, SVD compression,
+ansatz order, davidson(), energy_objective) is not available, so the restatements in
+oracle/glue.py and oracle/oracle.py drive them; the Pauli terms come from the reference's own
+generate_hamiltonian_real_64_128_f64.
+
+    python tests/golden/make_golden.py            # all fixtures (a few minutes on 8 cores)
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from binsim_amd.ansatz import ucc_spin_orbitals  # noqa: E402
+from binsim_amd.integrals import chemist_integrals, digest, for_norb  # noqa: E402
+from binsim_amd.integrals import one_body_layout, reference_layout  # noqa: E402
+from oracle import glue  # noqa: E402
+from oracle.ref import Reference  # noqa: E402
+
+SMALL = [(6, 3, 3), (6, 3, 2), (8, 4, 4), (8, 3, 5)]
+
+
+def seeded(dim, seed):
+    x = np.random.default_rng(seed).uniform(-1.0, 1.0, dim)
+    return x / np.linalg.norm(x)
+
+
+def ref_system(R, n, na, nb, h1, g2, pool=False):
+    terms = R.hamiltonian_terms(h1, g2, n)
+    flat = glue.flatten_network(glue.compress_by_svd(terms))
+    bp, dim = R.basis(n, na, nb)
+    net = R.network(flat, n)
+    out = dict(basis=bp, dim=dim, ham=net, hflat=flat, nterms=len(terms[4]))
+    if pool:
+        sos = ucc_spin_orbitals(n, na, nb)
+        pflat = glue.flatten_network(glue.compress_pool_by_svd([glue.ucc_generator(s) for s in sos]))
+        out.update(pool=R.network(pflat, n), sos=sos)
+    return out
+
+
+def small_fixtures(R):
+    for n, na, nb in SMALL:
+        h, eri = chemist_integrals(n, 7 + n)
+        h1, g2 = one_body_layout(h), reference_layout(eri)
+        S = ref_system(R, n, na, nb, h1, g2, pool=True)
+        dim = S["dim"]
+        x = seeded(dim, 11 + n)
+        sigma = R.hvec(S["basis"], S["ham"], x)
+        diag = R.diags(S["basis"], S["ham"], dim)
+        sos = S["sos"]
+        th = np.random.default_rng(n + na).uniform(-0.4, 0.4, len(sos))
+        v = np.zeros(dim)
+        v[0] = 1.0
+        for i, t in enumerate(th):
+            R.expm(S["basis"], S["pool"], i, t, v)
+        r = R.hvec(S["basis"], S["ham"], v)
+        grads = np.array([R.grad(S["basis"], S["pool"], i, 0.3, v, r) for i in range(len(sos))])
+        l2, r2 = v.copy(), r.copy()
+        back = np.zeros(len(sos))
+        for i in range(len(sos) - 1, -1, -1):
+            back[i] = R.backgrad(S["basis"], S["pool"], i, th[i], l2, r2)
+        np.savez_compressed(os.path.join(HERE, f"small_n{n}_{na}{nb}.npz"), seed=7 + n,
+                            x_seed=11 + n, sigma=sigma, diag=diag, thetas=th, state=v,
+                            grads_at_0p3=grads, backgrad=back, nterms=S["nterms"])
+        print("small", n, na, nb, "dim", dim, "pool", len(sos), flush=True)
+
+
+def davidson_ref(R, aop, v0, diags, tol):
+    """jl/davidson.jl:1-185 driving the reference's libdiag (oracle.oracle.davidson mirrors it)."""
+    import ctypes as C
+    dp = C.POINTER(C.c_double)
+    d = R.diag
+    N = len(v0)
+    ncv, maxspace = 3, 23
+    P = lambda a: a.ctypes.data_as(dp)  # noqa: E731
+    V, AV = [v0.copy()], [np.empty(N)]
+    d.inplace_normalize_f64(N, P(V[0]))
+    AV[0][:] = aop(V[0])
+    heff = np.zeros((maxspace, maxspace))
+    heff[0, 0] = d.fast_dot_f64(N, P(V[0]), P(AV[0]))
+    axt = np.empty(N)
+    dim = 1
+    for it in range(1, 100000):
+        es, vs = np.linalg.eigh(heff[:dim, :dim])
+        k = int(np.argmin(es))
+        e = float(es[k])
+        c = np.ascontiguousarray(vs[:, k])
+        nr = d.build_ritz_and_residual_f64(N, dim, P(c), e, R.ptr_array(V[:dim]),
+                                           R.ptr_array(AV[:dim]), P(axt))
+        if nr < tol:
+            return e, it
+        if dim >= maxspace:
+            order = np.argsort(es)
+            Vn = [sum(vs[j, order[i]] * V[j] for j in range(dim)) for i in range(ncv)]
+            AVn = [sum(vs[j, order[i]] * AV[j] for j in range(dim)) for i in range(ncv)]
+            for i in range(ncv):
+                V[i][:], AV[i][:] = Vn[i], AVn[i]
+            dim = ncv
+            d.rebuild_heff_f64(N, dim, R.ptr_array(V[:dim]), R.ptr_array(AV[:dim]), P(heff_c := np.zeros(maxspace * maxspace)), maxspace)
+            heff[:, :] = heff_c.reshape(maxspace, maxspace).T
+        shift = max(nr * 1e-2, 1e-4)
+        d.apply_preconditioner_inplace_f64(N, P(axt), P(diags), e, shift)
+        nb_ = float(np.linalg.norm(axt))
+        na_ = d.mgs_orthogonalize_f64(N, dim, R.ptr_array(V[:dim]), P(axt))
+        if na_ < 0.717 * nb_:
+            na_ = d.mgs_orthogonalize_f64(N, dim, R.ptr_array(V[:dim]), P(axt))
+        if na_ < 1e-12:
+            return e, it
+        d.inplace_normalize_f64(N, P(axt))
+        dim += 1
+        if len(V) < dim:
+            V.append(np.empty(N))
+            AV.append(np.empty(N))
+        V[dim - 1][:] = axt
+        AV[dim - 1][:] = aop(V[dim - 1])
+        col = np.zeros(dim)
+        d.compute_heff_col_f64(N, dim, R.ptr_array(V[:dim]), P(AV[dim - 1]), P(col))
+        heff[:dim, dim - 1] = col
+        heff[dim - 1, :dim] = col
+    raise RuntimeError("no convergence")
+
+
+def config0_fixtures(R):
+    n, na, nb = 12, 6, 6
+    h, eri = for_norb(n)
+    h1, g2 = one_body_layout(h), reference_layout(eri)
+    t0 = time.time()
+    S = ref_system(R, n, na, nb, h1, g2, pool=True)
+    dim = S["dim"]
+    x = seeded(dim, 1000 + 12)
+    t1 = time.time()
+    sigma = R.hvec(S["basis"], S["ham"], x)
+    t_sigma = time.time() - t1
+    diag = R.diags(S["basis"], S["ham"], dim)
+    y = seeded(dim, 4242)
+    rng = np.random.default_rng(99)
+    samp = np.sort(rng.choice(dim, 4096, replace=False))
+    hf = np.zeros(dim)
+    hf[0] = 1.0
+    t2 = time.time()
+    e0, iters = davidson_ref(R, lambda v: R.hvec(S["basis"], S["ham"], v), hf, diag, 1e-8)
+    t_dav = time.time() - t2
+    # VQE energy + gradient (config 2: all 1818 parameters, theta ~ U[-0.05, 0.05], seed+2000)
+    sos = S["sos"]
+    th = np.random.default_rng(20261202 + 2000).uniform(-0.05, 0.05, len(sos))
+    lv = hf.copy()
+    t3 = time.time()
+    for i, t in enumerate(th):
+        R.expm(S["basis"], S["pool"], i, t, lv)
+    rv = R.hvec(S["basis"], S["ham"], lv)
+    lnorm, rnorm = float(lv @ lv), float(rv @ rv)
+    e = float(lv @ rv) / lnorm
+    d2 = max(0.0, rnorm / lnorm - e * e)
+    g = np.zeros(len(sos))
+    for i in range(len(sos) - 1, -1, -1):
+        g[i] = R.backgrad(S["basis"], S["pool"], i, th[i], lv, rv) * 2.0 / lnorm
+    t_vqe = time.time() - t3
+    np.savez_compressed(os.path.join(HERE, "config0_n12.npz"), sigma_sum=sigma.sum(),
+                        sigma_norm2=sigma @ sigma, sigma_dot_y=y @ sigma, sample_idx=samp,
+                        sample_val=sigma[samp], diag_head=diag[:256], diag_sum=diag.sum(),
+                        e0=e0, davidson_iters=iters, vqe_thetas=th, vqe_energy=e, vqe_d2h=d2,
+                        vqe_grad=g, nterms=S["nterms"])
+    meta = dict(integrals_sha256=digest(h, eri), build_s=t1 - t0, sigma_s=t_sigma,
+                davidson_s=t_dav, davidson_iters=iters, e0=e0, vqe_s=t_vqe,
+                vqe_energy=e, threads=os.environ.get("OMP_NUM_THREADS", "default"))
+    with open(os.path.join(HERE, "config0_timing.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    print("config0", meta, flush=True)
+
+
+if __name__ == "__main__":
+    R = Reference()
+    small_fixtures(R)
+    config0_fixtures(R)
