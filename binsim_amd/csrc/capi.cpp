@@ -24,6 +24,16 @@ double dot(i64 N, const double *a, const double *b, hipStream_t st);
 void dot_dev(i64 N, const double *a, const double *b, double *dres, hipStream_t st);
 void multidot(i64 N, int dim, const double *const *V, const double *a, double *out_host, hipStream_t st);
 void validate_pool(int n, i64 nops, const int32_t *offsets, const int32_t *orbs);
+struct DavidsonOut {
+    int iterations = 0;
+    i64 sigma64 = 0, sigma32 = 0;
+    double max_residual = 0.0;
+    double seconds = 0.0;
+    int refine_iter = -1;
+    int converged = 0;
+};
+DavidsonOut davidson(DevHam *H, int nroots, int mode, double tol, double switch_tol, int maxiter,
+                     int maxspace, int verbose, const double *guess, double *energies, double *vecs);
 
 // ------------------------------------------------------------------------------------------
 // host network: the flat SVD-group arrays of build_network_otf (src/include/ham/otf.hpp:223-349)
@@ -524,6 +534,30 @@ int hipotf_vqe_energy_grad(void *basis, void *ham, void *pool, int64_t nparams, 
     out3[1] = std::max(0.0, rnorm / lnorm - e * e);
     out3[2] = lnorm;
     return 0;
+    API_CATCH(1)
+}
+
+// ---- Davidson driver (jl/davidson.jl outer loop on the device; see davidson.cpp) -----------
+// info (host, 7 doubles): iterations, fp64 sigma vectors, fp32 sigma vectors, max residual,
+// wall seconds, iteration of the fp32 -> fp64 switch (-1 if none), converged flag
+int hipotf_davidson_f64(void *basis, void *ham, int nroots, int mode, double tol, double switch_tol,
+                        int maxiter, int maxspace, int verbose, const double *guess, double *energies,
+                        double *vecs, double *info) {
+    API_TRY
+    HIPOTF_REQUIRE(energies && info, "NULL output");
+    HIPOTF_REQUIRE(mode == 0 || mode == 1, "mode must be 0 (fp64) or 1 (mixed block)");
+    DevBasis *B = as_basis(basis);
+    DevHam *H = as_net(ham)->ham_for(B);
+    DavidsonOut o = davidson(H, nroots, mode, tol, switch_tol, maxiter, maxspace, verbose, guess,
+                             energies, vecs);
+    info[0] = o.iterations;
+    info[1] = (double)o.sigma64;
+    info[2] = (double)o.sigma32;
+    info[3] = o.max_residual;
+    info[4] = o.seconds;
+    info[5] = o.refine_iter;
+    info[6] = o.converged;
+    return o.converged ? 0 : 2;
     API_CATCH(1)
 }
 

@@ -1,2 +1,291 @@
-// placeholder
+// Device-resident Davidson driver.
+//
+// mode 0 ("plain FP64"): the outer loop of jl/davidson.jl:1-185 -- Ritz pair from the small
+//   eigenproblem, residual, restart to the lowest Ritz vectors when the subspace is full,
+//   diagonal preconditioner with shift max(|r| 1e-2, 1e-4), modified Gram-Schmidt repeated
+//   when the norm drops below 0.717, normalisation -- extended to nroots roots (one correction
+//   vector per unconverged root per iteration, Davidson-Liu).  Every sigma is one FP64 vector.
+// mode 1 ("mixed block"): the same iteration with all correction vectors of an iteration
+//   applied in one block sigma launch in FP32 (coefficients, integrals and work vectors), the
+//   subspace, Rayleigh-Ritz, residuals and energies in FP64; when every residual is below
+//   switch_tol (or the FP32 residuals stop improving) the Ritz vectors seed an FP64 phase
+//   (FP64 block sigma) that runs to tol.
+// The small symmetric eigenproblem (m <= 64) is solved on the host by cyclic Jacobi.
 #include "hipotf.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <numeric>
+
+namespace hipotf {
+double ritz_and_residual(i64 N, int dim, const double *coeffs, double min_e, const double *const *V,
+                         const double *const *AV, double *axt, hipStream_t st);
+void ritz_vector(i64 N, int dim, const double *coeffs, const double *const *V, double *xt, hipStream_t st);
+void precondition(i64 N, double *axt, const double *diags, double min_e, double shift, hipStream_t st);
+double mgs(i64 N, int dim, const double *const *V, double *vt, hipStream_t st);
+double normalize(i64 N, double *vec, hipStream_t st);
+double dot(i64 N, const double *a, const double *b, hipStream_t st);
+void multidot(i64 N, int dim, const double *const *V, const double *a, double *out_host, hipStream_t st);
+
+// eigen-decomposition of a symmetric m x m matrix (row-major a), ascending eigenvalues,
+// eigenvectors in the columns of v (row-major)
+void jacobi_eigh(int m, std::vector<double> a, std::vector<double> &w, std::vector<double> &v) {
+    v.assign((size_t)m * m, 0.0);
+    for (int i = 0; i < m; ++i) v[i * m + i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int i = 0; i < m; ++i)
+            for (int j = 0; j < m; ++j) {
+                tot += a[i * m + j] * a[i * m + j];
+                if (i != j) off += a[i * m + j] * a[i * m + j];
+            }
+        if (off <= 1e-32 * tot || off == 0.0) break;
+        for (int p = 0; p < m - 1; ++p)
+            for (int q = p + 1; q < m; ++q) {
+                const double apq = a[p * m + q];
+                if (std::fabs(apq) < 1e-300) continue;
+                const double app = a[p * m + p], aqq = a[q * m + q];
+                const double theta = (aqq - app) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < m; ++k) {
+                    const double akp = a[k * m + p], akq = a[k * m + q];
+                    a[k * m + p] = c * akp - s * akq;
+                    a[k * m + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < m; ++k) {
+                    const double apk = a[p * m + k], aqk = a[q * m + k];
+                    a[p * m + k] = c * apk - s * aqk;
+                    a[q * m + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < m; ++k) {
+                    const double vkp = v[k * m + p], vkq = v[k * m + q];
+                    v[k * m + p] = c * vkp - s * vkq;
+                    v[k * m + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    std::vector<int> order(m);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int i, int j) { return a[i * m + i] < a[j * m + j]; });
+    w.resize(m);
+    std::vector<double> vs((size_t)m * m);
+    for (int c = 0; c < m; ++c) {
+        w[c] = a[order[c] * m + order[c]];
+        for (int r = 0; r < m; ++r) vs[r * m + c] = v[r * m + order[c]];
+    }
+    v.swap(vs);
+}
+
+struct DavidsonOut {
+    int iterations = 0;
+    i64 sigma64 = 0, sigma32 = 0;
+    double max_residual = 0.0;
+    double seconds = 0.0;
+    int refine_iter = -1;
+    int converged = 0;
+};
+
+DavidsonOut davidson(DevHam *H, int nroots, int mode, double tol, double switch_tol, int maxiter,
+                     int maxspace, int verbose, const double *guess, double *energies, double *vecs) {
+    const auto t0 = std::chrono::steady_clock::now();
+    hipStream_t st = (hipStream_t)0;
+    const i64 N = H->basis->host.dim;
+    HIPOTF_REQUIRE(N > 0, "empty basis");
+    HIPOTF_REQUIRE(nroots >= 1 && nroots <= 16 && nroots <= N, "nroots out of range");
+    if (maxspace <= 0) maxspace = nroots == 1 ? 23 : 8 * nroots;
+    maxspace = (int)std::min<i64>(std::min(maxspace, 64), N);
+    const int nkeep = std::max(nroots, std::min(3 * nroots, maxspace / 2));  // ncv = 3 for one root
+    HIPOTF_REQUIRE(maxspace >= 2 * nroots || maxspace == N, "maxspace must be >= 2 * nroots");
+    DavidsonOut out;
+
+    std::vector<DevBuf> Vb(maxspace), AVb(maxspace), R(nroots);
+    for (auto &b : Vb) b.alloc(N * sizeof(double));
+    for (auto &b : AVb) b.alloc(N * sizeof(double));
+    for (auto &b : R) b.alloc(N * sizeof(double));
+    DevBuf diag(N * sizeof(double)), tmp(N * sizeof(double) * (size_t)nroots);
+    HIPOTF_CHECK(hipMemsetAsync(diag.p, 0, N * sizeof(double), st));
+    diag_elements(H, diag.as<double>(), st);
+    std::vector<double *> V(maxspace), AV(maxspace);
+    for (int i = 0; i < maxspace; ++i) {
+        V[i] = Vb[i].as<double>();
+        AV[i] = AVb[i].as<double>();
+    }
+
+    // initial guesses: caller's vectors, else unit vectors on the nroots lowest diagonals
+    if (guess) {
+        for (int i = 0; i < nroots; ++i)
+            HIPOTF_CHECK(hipMemcpyAsync(V[i], guess + (size_t)i * N, N * sizeof(double),
+                                        hipMemcpyDeviceToDevice, st));
+    } else {
+        std::vector<double> dh(N);
+        HIPOTF_CHECK(hipMemcpy(dh.data(), diag.p, N * sizeof(double), hipMemcpyDeviceToHost));
+        std::vector<i64> idx(N);
+        std::iota(idx.begin(), idx.end(), 0);
+        std::partial_sort(idx.begin(), idx.begin() + nroots, idx.end(),
+                          [&](i64 a, i64 b) { return dh[a] < dh[b] || (dh[a] == dh[b] && a < b); });
+        const double one = 1.0;
+        for (int i = 0; i < nroots; ++i) {
+            HIPOTF_CHECK(hipMemsetAsync(V[i], 0, N * sizeof(double), st));
+            HIPOTF_CHECK(hipMemcpyAsync(V[i] + idx[i], &one, sizeof(double), hipMemcpyHostToDevice, st));
+            HIPOTF_CHECK(hipStreamSynchronize(st));
+        }
+    }
+    int m = 0;
+    for (int i = 0; i < nroots; ++i) {
+        double nb = std::sqrt(dot(N, V[i], V[i], st));
+        double na = mgs(N, m, (const double *const *)V.data(), V[i], st);
+        if (na < 0.717 * nb) na = mgs(N, m, (const double *const *)V.data(), V[i], st);
+        HIPOTF_REQUIRE(na > 1e-12, "initial guesses are linearly dependent");
+        normalize(N, V[i], st);
+        ++m;
+    }
+    int phase = mode == 1 ? 1 : 0;  // 1: fp32 block sigma, 0: fp64
+    auto apply = [&](int first, int count) {
+        if (count == 0) return;
+        if (mode == 0) {
+            for (int i = first; i < first + count; ++i) {
+                const double *s = V[i];
+                double *d = AV[i];
+                sigma(H, 1, &s, &d, 0, st);
+                out.sigma64++;
+            }
+        } else {
+            sigma(H, count, (const double *const *)&V[first], &AV[first], phase, st);
+            if (phase == 1) out.sigma32 += count;
+            else out.sigma64 += count;
+        }
+    };
+    apply(0, m);
+    std::vector<double> heff((size_t)maxspace * maxspace, 0.0);
+    auto set_cols = [&](int first, int count) {
+        std::vector<double> col(maxspace);
+        for (int c = first; c < first + count; ++c) {
+            multidot(N, c + 1, (const double *const *)V.data(), AV[c], col.data(), st);
+            for (int r = 0; r <= c; ++r) heff[(size_t)r * maxspace + c] = heff[(size_t)c * maxspace + r] = col[r];
+        }
+    };
+    set_cols(0, m);
+
+    std::vector<double> w, C, best(3, 1e300);
+    std::vector<double> rn(nroots);
+    int stall = 0;
+    for (int it = 1; it <= maxiter; ++it) {
+        out.iterations = it;
+        std::vector<double> a((size_t)m * m);
+        for (int r = 0; r < m; ++r)
+            for (int c = 0; c < m; ++c) a[(size_t)r * m + c] = heff[(size_t)r * maxspace + c];
+        jacobi_eigh(m, a, w, C);
+        std::vector<std::vector<double>> coef(nroots, std::vector<double>(m));
+        double rmax = 0.0;
+        for (int i = 0; i < nroots; ++i) {
+            for (int j = 0; j < m; ++j) coef[i][j] = C[(size_t)j * m + i];
+            rn[i] = ritz_and_residual(N, m, coef[i].data(), w[i], (const double *const *)V.data(),
+                                      (const double *const *)AV.data(), R[i].as<double>(), st);
+            rmax = std::max(rmax, rn[i]);
+        }
+        out.max_residual = rmax;
+        if (verbose) {
+            std::printf("  Step %03d [%s] m=%2d  max residual: %.4e  e0: %.14f\n", it,
+                        phase == 1 ? "fp32" : "fp64", m, rmax, w[0]);
+            std::fflush(stdout);
+        }
+        const double cur_tol = phase == 1 ? std::max(switch_tol, tol) : tol;
+        // FP32 phase: leave when converged to switch_tol or when the residual stagnates
+        if (phase == 1) {
+            if (rmax < 0.9 * best[0]) {
+                best[0] = rmax;
+                stall = 0;
+            } else if (++stall >= 4 && rmax < 1e-3) {
+                stall = 0;
+                rmax = 0.0;  // force the switch
+            }
+        }
+        if (rmax < cur_tol) {
+            if (phase == 1) {
+                // Ritz vectors -> new subspace, FP64 sigma from here on
+                for (int i = 0; i < nroots; ++i)
+                    ritz_vector(N, m, coef[i].data(), (const double *const *)V.data(),
+                                tmp.as<double>() + (size_t)i * N, st);
+                for (int i = 0; i < nroots; ++i)
+                    HIPOTF_CHECK(hipMemcpyAsync(V[i], tmp.as<double>() + (size_t)i * N, N * sizeof(double),
+                                                hipMemcpyDeviceToDevice, st));
+                m = 0;
+                for (int i = 0; i < nroots; ++i) {
+                    mgs(N, m, (const double *const *)V.data(), V[i], st);
+                    normalize(N, V[i], st);
+                    ++m;
+                }
+                phase = 0;
+                out.refine_iter = it;
+                apply(0, m);
+                set_cols(0, m);
+                continue;
+            }
+            for (int i = 0; i < nroots; ++i) energies[i] = w[i];
+            if (vecs)
+                for (int i = 0; i < nroots; ++i)
+                    ritz_vector(N, m, coef[i].data(), (const double *const *)V.data(), vecs + (size_t)i * N, st);
+            out.converged = 1;
+            break;
+        }
+        // correction vectors for the unconverged roots
+        std::vector<int> todo;
+        for (int i = 0; i < nroots; ++i)
+            if (rn[i] >= cur_tol) todo.push_back(i);
+        if (m + (int)todo.size() > maxspace) {
+            // restart to the nkeep lowest Ritz vectors (jl/davidson.jl:82-114)
+            const int keep = std::min(nkeep, m);
+            std::vector<std::vector<double>> ck(keep, std::vector<double>(m));
+            for (int i = 0; i < keep; ++i)
+                for (int j = 0; j < m; ++j) ck[i][j] = C[(size_t)j * m + i];
+            DevBuf nv((size_t)keep * N * sizeof(double)), nav((size_t)keep * N * sizeof(double));
+            for (int i = 0; i < keep; ++i) {
+                ritz_vector(N, m, ck[i].data(), (const double *const *)V.data(), nv.as<double>() + (size_t)i * N, st);
+                ritz_vector(N, m, ck[i].data(), (const double *const *)AV.data(), nav.as<double>() + (size_t)i * N, st);
+            }
+            for (int i = 0; i < keep; ++i) {
+                HIPOTF_CHECK(hipMemcpyAsync(V[i], nv.as<double>() + (size_t)i * N, N * sizeof(double),
+                                            hipMemcpyDeviceToDevice, st));
+                HIPOTF_CHECK(hipMemcpyAsync(AV[i], nav.as<double>() + (size_t)i * N, N * sizeof(double),
+                                            hipMemcpyDeviceToDevice, st));
+            }
+            m = keep;
+            std::fill(heff.begin(), heff.end(), 0.0);
+            set_cols(0, m);
+        }
+        const int first = m;
+        for (int i : todo) {
+            if (m >= maxspace) break;
+            const double shift = std::max(rn[i] * 1e-2, 1e-4);
+            double *t = V[m];
+            HIPOTF_CHECK(hipMemcpyAsync(t, R[i].p, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+            precondition(N, t, diag.as<double>(), w[i], shift, st);
+            const double nb = std::sqrt(dot(N, t, t, st));
+            double na = mgs(N, m, (const double *const *)V.data(), t, st);
+            if (na < 0.717 * nb) na = mgs(N, m, (const double *const *)V.data(), t, st);
+            if (na < 1e-12 * std::max(1.0, nb)) continue;  // lies in the subspace
+            normalize(N, t, st);
+            ++m;
+        }
+        if (m == first) {
+            // breakdown: no new direction (jl/davidson.jl:135-143)
+            for (int i = 0; i < nroots; ++i) energies[i] = w[i];
+            if (vecs)
+                for (int i = 0; i < nroots; ++i)
+                    ritz_vector(N, std::min(m, (int)coef[i].size()), coef[i].data(),
+                                (const double *const *)V.data(), vecs + (size_t)i * N, st);
+            out.converged = 2;
+            break;
+        }
+        apply(first, m - first);
+        set_cols(first, m - first);
+    }
+    HIPOTF_CHECK(hipDeviceSynchronize());
+    out.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return out;
+}
+
+}  // namespace hipotf

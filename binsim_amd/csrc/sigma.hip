@@ -212,6 +212,117 @@ __global__ __launch_bounds__(1024) void k_ab(const T *__restrict__ xt, T *__rest
 }
 
 // ---------------------------------------------------------------------------------------
+// K4 (register variant): one beta element per lane, its nsxB single-excitation descriptors
+// held in VGPRs for the whole sweep over the alpha connections, so the connectivity table is
+// read once per workgroup instead of once per staged source row.  A destination row is
+// covered by ceil(NB / blockDim) workgroups that run back to back on one XCD, so the source
+// rows they all stage come from that XCD's L2.
+// ---------------------------------------------------------------------------------------
+template <typename T, int NVS, int SXMAX>
+__global__ __launch_bounds__(1024) void k_ab_reg(const T *__restrict__ xt, T *__restrict__ s1,
+                                                 int nvst, int v0, const u32 *__restrict__ sxA,
+                                                 const u32 *__restrict__ sxB,
+                                                 const u64 *__restrict__ strsA,
+                                                 const u64 *__restrict__ strsB,
+                                                 const T *__restrict__ eri, int n, i64 NA, i64 NB,
+                                                 int nsxA, int nsxB, int nsplit) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *W = reinterpret_cast<T *>(smem_raw);
+    const int n2 = n * n;
+    T *Wd = W + n2;                       // W[r*n + r], r < n
+    T *xrow = W + ((n2 + n + 15) & ~15);
+    const int tid = threadIdx.x, bd = blockDim.x;
+    const i64 per = (NA + 7) / 8;
+    const i64 jx = blockIdx.x >> 3;
+    const i64 Ia = (blockIdx.x & 7) * per + jx / nsplit;
+    if (Ia >= NA) return;
+    const i64 Ib = (jx % nsplit) * (i64)bd + tid;
+    const bool live = Ib < NB;
+    const u64 astr = strsA[Ia];
+
+    u32 d[SXMAX];
+#pragma unroll
+    for (int c = 0; c < SXMAX; ++c) {
+        u32 e = (live && c < nsxB) ? sxB[(i64)c * NB + Ib] : 0u;
+        d[c] = sx_J(e) | (u32(sx_r(e) * n + sx_s(e)) << 20) | (e & 0x80000000u);
+    }
+    const u64 bocc = live ? strsB[Ib] : 0ull;
+
+    T acc[NVS];
+#pragma unroll
+    for (int v = 0; v < NVS; ++v) acc[v] = 0;
+
+    for (int ca = 0; ca <= nsxA; ++ca) {
+        i64 Ja;
+        __syncthreads();
+        if (ca < nsxA) {
+            const u32 e = sxA[(i64)ca * NA + Ia];
+            Ja = sx_J(e);
+            const T sg = sx_neg(e) ? T(-1) : T(1);
+            const T *er = eri + (i64)(sx_r(e) * n + sx_s(e)) * n2;
+            for (int t = tid; t < n2; t += bd) W[t] = sg * er[t];
+            for (int t = tid; t < n; t += bd) Wd[t] = sg * er[t * n + t];
+        } else {
+            Ja = Ia;
+            for (int t = tid; t < n2 + n; t += bd) {
+                const int tt = t < n2 ? t : (t - n2) * (n + 1);
+                T v = 0;
+                u64 m = astr;
+                while (m) {
+                    const int p = __builtin_ctzll(m);
+                    m &= m - 1;
+                    v += eri[(i64)(p * n + p) * n2 + tt];
+                }
+                W[t] = v;  // t >= n2 lands in Wd[t - n2]
+            }
+        }
+        const T *srow = xt + Ja * NB * nvst + v0;
+        if (NVS == 1 && nvst == 1) {
+            for (i64 t = tid; t < NB; t += bd) xrow[t] = srow[t];
+        } else {
+            for (i64 t = tid; t < NB * NVS; t += bd) {
+                const i64 el = t / NVS;
+                xrow[t] = srow[el * nvst + (t - el * NVS)];
+            }
+        }
+        __syncthreads();
+        if (!live) continue;
+        T a[NVS];
+#pragma unroll
+        for (int v = 0; v < NVS; ++v) a[v] = 0;
+#pragma unroll
+        for (int c = 0; c < SXMAX; ++c) {
+            if (c < nsxB) {
+                // lane-ready descriptor: J (20 bits) | r*n+s (10 bits) << 20 | sign << 31.
+                // The empty asm keeps the decode inside the loop: hoisting it would cost two
+                // VGPRs per slot and spill.
+                u32 e = d[c];
+                asm volatile("" : "+v"(e));
+                T w = W[(e >> 20) & 0x3FFu];
+                if (e >> 31) w = -w;
+                const T *xp = xrow + (e & 0xFFFFFu) * NVS;
+#pragma unroll
+                for (int v = 0; v < NVS; ++v) a[v] += w * xp[v];
+            }
+        }
+        T wd = 0;
+        u64 m = bocc;
+        while (m) {
+            const int r = __builtin_ctzll(m);
+            m &= m - 1;
+            wd += Wd[r];
+        }
+        const T *xd = xrow + Ib * NVS;
+#pragma unroll
+        for (int v = 0; v < NVS; ++v) acc[v] += a[v] + wd * xd[v];
+    }
+    if (live) {
+#pragma unroll
+        for (int v = 0; v < NVS; ++v) s1[(Ia * NB + Ib) * nvst + v0 + v] += acc[v];
+    }
+}
+
+// ---------------------------------------------------------------------------------------
 // K5: sigma = E (s1 + s2T^T + E_core x~), FP64 output vectors
 // ---------------------------------------------------------------------------------------
 template <typename T, int NV>
@@ -319,6 +430,33 @@ static void launch_ab_nvs(DevHam *H, const T *xt, T *s1, int nvst, int v0, const
     DevBasis *B = H->basis;
     const i64 NB = B->pB->N;
     const int n2 = H->n * H->n;
+    // register-descriptor kernel whenever a whole source row fits LDS
+    {
+        const DevSpin &A = *B->pA, &Bs = *B->pB;
+        const size_t wb = (size_t)((n2 + H->n + 15) & ~15) * sizeof(T);
+        const size_t smem = wb + (size_t)NB * NVS * sizeof(T);
+        const size_t budget = std::min<size_t>(lds_budget(), 160 * 1024) - 1024;
+        if (smem <= budget && Bs.nsx <= 80) {
+            const int bd = NB >= 1024 ? 1024 : (int)std::max<i64>(64, (NB + 63) / 64 * 64);
+            const int nsplit = (int)((NB + bd - 1) / bd);
+            const i64 per = (A.N + 7) / 8;
+            const i64 nblocks = per * nsplit * 8;
+            HIPOTF_REQUIRE(nblocks < (1ll << 31), "alpha-beta grid too large");
+            auto go = [&](auto kern) {
+                HIPOTF_CHECK(hipFuncSetAttribute((const void *)kern,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+                hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(bd), smem, st, xt, s1, nvst, v0,
+                                   A.sx.as<u32>(), Bs.sx.as<u32>(), A.strs.as<u64>(), Bs.strs.as<u64>(),
+                                   eri, H->n, A.N, Bs.N, A.nsx, Bs.nsx, nsplit);
+                HIPOTF_CHECK(hipGetLastError());
+            };
+            if (Bs.nsx <= 16) go(k_ab_reg<T, NVS, 16>);
+            else if (Bs.nsx <= 40) go(k_ab_reg<T, NVS, 40>);
+            else if (Bs.nsx <= 64) go(k_ab_reg<T, NVS, 64>);
+            else go(k_ab_reg<T, NVS, 80>);
+            return;
+        }
+    }
     const size_t wbytes = (size_t)((n2 + 15) & ~15) * sizeof(T);
     const size_t budget = std::min<size_t>(lds_budget(), 160 * 1024) - 1024;
     HIPOTF_REQUIRE(budget > wbytes + 64 * NVS * sizeof(T), "LDS too small for the alpha-beta kernel");

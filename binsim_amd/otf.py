@@ -46,7 +46,7 @@ class BasisManager:
         self.dim = int(lib.get_subspace_dim(self.ptr))
 
     def __del__(self):
-        if getattr(self, "ptr", None):
+        if getattr(self, "ptr", None) and L is not None and L.load is not None:
             L.load().destroy_basis_manager(self.ptr)
             self.ptr = None
 
@@ -69,7 +69,7 @@ class HipBasisManager:
         self.dim = basis.dim
 
     def __del__(self):
-        if getattr(self, "ptr", None):
+        if getattr(self, "ptr", None) and L is not None and L.load is not None:
             L.load().destroy_basisdev_f64(self.ptr)
             self.ptr = None
 
@@ -125,7 +125,7 @@ class HipOTF:
         return cls(p, "network")
 
     def __del__(self):
-        if getattr(self, "ptr", None):
+        if getattr(self, "ptr", None) and L is not None and L.load is not None:
             L.load().destroy_networkdev_f64(self.ptr)
             self.ptr = None
 
