@@ -97,7 +97,8 @@ DavidsonOut davidson(DevHam *H, int nroots, int mode, double tol, double switch_
     HIPOTF_REQUIRE(nroots >= 1 && nroots <= 16 && nroots <= N, "nroots out of range");
     if (maxspace <= 0) maxspace = nroots == 1 ? 23 : 8 * nroots;
     maxspace = (int)std::min<i64>(std::min(maxspace, 64), N);
-    const int nkeep = std::max(nroots, std::min(3 * nroots, maxspace / 2));  // ncv = 3 for one root
+    // vectors kept on restart: ncv = 3 for one root (jl/davidson.jl), 4 per root otherwise
+    const int nkeep = nroots == 1 ? std::min(3, maxspace - 1) : std::max(nroots, std::min(4 * nroots, maxspace / 2));
     HIPOTF_REQUIRE(maxspace >= 2 * nroots || maxspace == N, "maxspace must be >= 2 * nroots");
     DavidsonOut out;
 
@@ -187,10 +188,11 @@ DavidsonOut davidson(DevHam *H, int nroots, int mode, double tol, double switch_
             rmax = std::max(rmax, rn[i]);
         }
         out.max_residual = rmax;
-        if (verbose) {
-            std::printf("  Step %03d [%s] m=%2d  max residual: %.4e  e0: %.14f\n", it,
-                        phase == 1 ? "fp32" : "fp64", m, rmax, w[0]);
-            std::fflush(stdout);
+        if (verbose) {  // progress goes to stderr: stdout belongs to the caller
+            const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            std::fprintf(stderr, "  Step %03d [%s] m=%2d  max residual: %.4e  e0: %.14f  t=%.1fs\n", it,
+                         phase == 1 ? "fp32" : "fp64", m, rmax, w[0], el);
+            std::fflush(stderr);
         }
         const double cur_tol = phase == 1 ? std::max(switch_tol, tol) : tol;
         // FP32 phase: leave when converged to switch_tol or when the residual stagnates
