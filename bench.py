@@ -8,9 +8,10 @@ seeded vector; ``value`` is its algorithmic HBM rate  B_sigma / t  with
 B_sigma = 8 * dim * (2 + s_a + d_a) bytes (s_a = 64 singles, d_a = 784 doubles per string).
 
 Reported beside it (one run each, outside the timed steps):
-  * FCI, lowest 4 roots to residual 1e-8: plain FP64 Davidson (one FP64 sigma per vector) and
-    mixed-precision block Davidson (FP32 block sigma, FP64 Rayleigh-Ritz, FP64 refinement),
-    wall seconds and per-root |dE|;
+  * FCI, lowest 4 roots: plain FP64 Davidson (one FP64 sigma per vector) and mixed-precision
+    block Davidson (FP32 block sigma, FP64 Rayleigh-Ritz, FP64 refinement), wall seconds and
+    per-root |dE| -- at 12 orbitals / 6+6 electrons to residual 1e-8 by default, and at the
+    16-orbital workload to residual 1e-6 with --full (each solve takes minutes there);
   * FCI at 12 orbitals / 6+6 electrons, one root from HF to 1e-8 (the CPU-runnable case);
   * VQE energy + adjoint gradient (fused) at 12 orbitals (1,818 parameters) and at 16 orbitals
     (5,792 parameters): gradients/s and max deviation from central differences (h = 1e-6);
@@ -79,11 +80,12 @@ def timed_sigma(torch, F, x, y, steps, warmup):
     return t0.elapsed_time(t1) / steps, wall
 
 
-def fci_block(torch, B, H, nroots, quick):
+def fci_block(torch, B, H, nroots, maxspace, tol, maxiter):
     from binsim_amd import davidson as D
-    out = {}
+    out = {"tol": tol, "maxspace": maxspace}
     for name, mixed in (("plain_fp64", False), ("mixed_block", True)):
-        e, _, info = D.solve(B, H, nroots=nroots, mixed=mixed, tol=1e-8, maxiter=300 if not quick else 60)
+        e, _, info = D.solve(B, H, nroots=nroots, mixed=mixed, tol=tol, maxiter=maxiter,
+                             maxspace=maxspace, verbose=True)
         out[name] = dict(seconds=round(info["seconds"], 3), iterations=int(info["iterations"]),
                          sigma_fp64=int(info["sigma64"]), sigma_fp32=int(info["sigma32"]),
                          converged=bool(info["converged"] == 1),
@@ -174,6 +176,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--quick", action="store_true", help="skip the FCI solves and VQE checks")
+    ap.add_argument("--full", action="store_true",
+                    help="also run the 4-root FCI solves at 16 orbitals (plain and mixed; ~15 min)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -221,7 +225,8 @@ def main():
         "setup_s": round(setup_s, 3), "wall_s_timed": round(wall, 3),
     }
     if not args.quick:
-        result["fci_16o_4roots"] = fci_block(torch, B, H, 4, args.quick)
+        if args.full:
+            result["fci_16o_4roots"] = fci_block(torch, B, H, 4, maxspace=64, tol=1e-6, maxiter=400)
         del B, H, F, x, y
         torch.cuda.empty_cache()
         from binsim_amd import davidson as D
@@ -229,8 +234,9 @@ def main():
         hf = torch.zeros(B0.dim, dtype=torch.float64, device="cuda")
         hf[0] = 1.0
         e, _, info = D.solve(B0, H0, nroots=1, mixed=False, tol=1e-8, guess=hf)
-        result["fci_12o_1root"] = dict(seconds=round(info["seconds"], 4), energy=float(e[0]),
-                                       iterations=int(info["iterations"]))
+        result["fci_12o_1root_from_hf"] = dict(seconds=round(info["seconds"], 4), energy=float(e[0]),
+                                               iterations=int(info["iterations"]))
+        result["fci_12o_4roots"] = fci_block(torch, B0, H0, 4, maxspace=32, tol=1e-8, maxiter=400)
         del B0, H0
         result["vqe_12o"] = vqe(torch, 12, 6, 6, nfd=8)
         torch.cuda.empty_cache()

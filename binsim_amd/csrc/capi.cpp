@@ -515,7 +515,7 @@ int hipotf_vqe_energy_grad(void *basis, void *ham, void *pool, int64_t nparams, 
     HIPOTF_REQUIRE(psi && lam && grads && out3 && (nparams == 0 || (idxs && thetas)), "NULL argument");
     const i64 dim = B->host.dim;
     hipStream_t st = S0();
-    for (i64 i = 0; i < nparams; ++i) rot_expm(P, idxs[i], thetas[i], psi, st);
+    fused_sweeps(P, nparams, idxs, thetas, psi, lam, true, false, nullptr, st);
     const double *src = psi;
     double *dst = lam;
     sigma(H, 1, &src, &dst, 0, st);
@@ -524,7 +524,7 @@ int hipotf_vqe_energy_grad(void *basis, void *ham, void *pool, int64_t nparams, 
     dot_dev(dim, psi, psi, d + nparams, st);
     dot_dev(dim, lam, lam, d + nparams + 1, st);
     dot_dev(dim, psi, lam, d + nparams + 2, st);
-    for (i64 i = nparams - 1; i >= 0; --i) rot_backgrad(P, idxs[i], thetas[i], psi, lam, d + i, st);
+    fused_sweeps(P, nparams, idxs, thetas, psi, lam, false, true, d, st);
     std::vector<double> h(nparams + 3);
     HIPOTF_CHECK(hipMemcpyAsync(h.data(), d, (nparams + 3) * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPOTF_CHECK(hipStreamSynchronize(st));

@@ -193,7 +193,14 @@ struct DevPool {
     std::vector<PairList> lists;   // lists[0] = alpha identity, lists[1] = beta identity
     DevBuf partial;                // double partial sums for reductions
     DevBuf scal;                   // a few device doubles
+    DevBuf segdesc, segpart;       // fused sweeps: per-position descriptors, per-block partials
+    DevBuf tl, tr;                 // fused sweeps: beta-major copies of the two state vectors
 };
+
+// forward expm sweep and/or adjoint backgrad sweep over positions idxs[0..n) with fused
+// row-local segments; dres[i] = backgrad value of position i (device)
+void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, double *psi, double *lam,
+                  bool forward, bool backward, double *dres, hipStream_t st);
 
 DevPool *make_dev_pool(DevBasis *b, i64 nops, const int32_t *offsets, const int32_t *orbs);
 

@@ -331,6 +331,344 @@ void rot_backgrad(DevPool *P, i64 idx, double theta, double *lp, double *rp, dou
 }
 
 // ---------------------------------------------------------------------------------------
+// Fused sweeps.  Consecutive generators that only move amplitudes inside the same rows are
+// applied by one launch with the rows staged in LDS:
+//   ROW_A : pure-beta generators act inside each alpha row (alpha-major layout);
+//   ROW_B : pure-alpha generators act inside each beta row of the beta-major copy;
+//   PAIR  : generators sharing one alpha excitation pattern (an alpha single and the
+//           alpha-beta doubles built on it) act inside each row pair (u_a, w_a) of that pattern.
+// The rotations of a segment are applied in sweep order (reverse order in the adjoint sweep),
+// so the result equals the per-generator launches; gradients are reduced per generator from
+// per-block partials in fixed order.
+// ---------------------------------------------------------------------------------------
+struct SegGen {
+    GenDev g;
+    const int32_t *lst;  // (u, w) pairs over the elements of a staged row
+    i64 len;
+    double c, s;
+};
+
+enum { SEG_ROW_A = 0, SEG_ROW_B = 1, SEG_PAIR = 2, SEG_SINGLE = 3 };
+
+// one block = one row of nel elements; BACK: adjoint step on (l, r) with gradient partials
+template <bool BACK>
+__global__ __launch_bounds__(256) void k_seg_row(double *__restrict__ l, double *__restrict__ r,
+                                                 i64 nrows, i64 nel, const u64 *__restrict__ row_strs,
+                                                 const u64 *__restrict__ el_strs, int row_is_alpha,
+                                                 const SegGen *__restrict__ gens, int ng,
+                                                 double *__restrict__ partial) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double *xl = reinterpret_cast<double *>(smem_raw);
+    double *xr = xl + nel;
+    const i64 row = blockIdx.x;
+    if (row >= nrows) return;
+    const int tid = threadIdx.x, bd = blockDim.x;
+    const u64 rs = row_strs[row];
+    double *gl = l + row * nel;
+    double *gr = BACK ? r + row * nel : nullptr;
+    for (i64 t = tid; t < nel; t += bd) {
+        xl[t] = gl[t];
+        if (BACK) xr[t] = gr[t];
+    }
+    __syncthreads();
+    for (int kk = 0; kk < ng; ++kk) {
+        const int k = BACK ? ng - 1 - kk : kk;
+        const SegGen sg = gens[k];
+        double acc = 0.0;
+        for (i64 j = tid; j < sg.len; j += bd) {
+            const int32_t u = sg.lst[2 * j], w = sg.lst[2 * j + 1];
+            const u64 es = el_strs[u];
+            const bool neg = row_is_alpha ? jw_sign(sg.g, rs, es) : jw_sign(sg.g, es, rs);
+            const double ts = neg ? -sg.s : sg.s;
+            if (!BACK) {
+                const double xu = xl[u], xw = xl[w];
+                xl[u] = sg.c * xu - ts * xw;
+                xl[w] = ts * xu + sg.c * xw;
+            } else {
+                const double tc = neg ? -sg.c : sg.c;
+                const double lu = xl[u], lw = xl[w], ru = xr[u], rw = xr[w];
+                const double nlu = sg.c * lu + ts * lw, nlw = -ts * lu + sg.c * lw;
+                acc += nlu * (-sg.s * ru + tc * rw) + nlw * (-sg.s * rw - tc * ru);
+                xl[u] = nlu;
+                xl[w] = nlw;
+                xr[u] = sg.c * ru + ts * rw;
+                xr[w] = -ts * ru + sg.c * rw;
+            }
+        }
+        if (BACK) {
+            const double t = block_sum(acc);
+            if (tid == 0) partial[(i64)k * nrows + row] = t;
+        }
+        __syncthreads();
+    }
+    for (i64 t = tid; t < nel; t += bd) {
+        gl[t] = xl[t];
+        if (BACK) gr[t] = xr[t];
+    }
+}
+
+// one block = one alpha pair (u_a, w_a) of the segment's alpha pattern; rows of NB elements
+template <bool BACK>
+__global__ __launch_bounds__(256) void k_seg_pair(double *__restrict__ l, double *__restrict__ r,
+                                                  const int32_t *__restrict__ alist, i64 npairs, i64 NB,
+                                                  const u64 *__restrict__ strsA,
+                                                  const u64 *__restrict__ strsB,
+                                                  const SegGen *__restrict__ gens, int ng,
+                                                  double *__restrict__ partial) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double *l0 = reinterpret_cast<double *>(smem_raw);
+    double *l1 = l0 + NB;
+    double *r0 = l1 + NB;
+    double *r1 = r0 + NB;
+    const i64 p = blockIdx.x;
+    if (p >= npairs) return;
+    const int tid = threadIdx.x, bd = blockDim.x;
+    const int32_t ua = alist[2 * p], wa = alist[2 * p + 1];
+    const u64 as = strsA[ua];
+    for (i64 t = tid; t < NB; t += bd) {
+        l0[t] = l[(i64)ua * NB + t];
+        l1[t] = l[(i64)wa * NB + t];
+        if (BACK) {
+            r0[t] = r[(i64)ua * NB + t];
+            r1[t] = r[(i64)wa * NB + t];
+        }
+    }
+    __syncthreads();
+    for (int kk = 0; kk < ng; ++kk) {
+        const int k = BACK ? ng - 1 - kk : kk;
+        const SegGen sg = gens[k];
+        double acc = 0.0;
+        for (i64 j = tid; j < sg.len; j += bd) {
+            const int32_t ub = sg.lst[2 * j], wb = sg.lst[2 * j + 1];
+            const bool neg = jw_sign(sg.g, as, strsB[ub]);
+            const double ts = neg ? -sg.s : sg.s;
+            if (!BACK) {
+                const double xu = l0[ub], xw = l1[wb];
+                l0[ub] = sg.c * xu - ts * xw;
+                l1[wb] = ts * xu + sg.c * xw;
+            } else {
+                const double tc = neg ? -sg.c : sg.c;
+                const double lu = l0[ub], lw = l1[wb], ru = r0[ub], rw = r1[wb];
+                const double nlu = sg.c * lu + ts * lw, nlw = -ts * lu + sg.c * lw;
+                acc += nlu * (-sg.s * ru + tc * rw) + nlw * (-sg.s * rw - tc * ru);
+                l0[ub] = nlu;
+                l1[wb] = nlw;
+                r0[ub] = sg.c * ru + ts * rw;
+                r1[wb] = -ts * ru + sg.c * rw;
+            }
+        }
+        if (BACK) {
+            const double t = block_sum(acc);
+            if (tid == 0) partial[(i64)k * npairs + p] = t;
+        }
+        __syncthreads();
+    }
+    for (i64 t = tid; t < NB; t += bd) {
+        l[(i64)ua * NB + t] = l0[t];
+        l[(i64)wa * NB + t] = l1[t];
+        if (BACK) {
+            r[(i64)ua * NB + t] = r0[t];
+            r[(i64)wa * NB + t] = r1[t];
+        }
+    }
+}
+
+// dres[k] = sum_b partial[k * nb + b], one block per generator of the segment
+__global__ __launch_bounds__(256) void k_seg_reduce(const double *__restrict__ partial, i64 nb,
+                                                    double *__restrict__ dres) {
+    const int k = blockIdx.x;
+    double acc = 0.0;
+    for (i64 i = threadIdx.x; i < nb; i += blockDim.x) acc += partial[(i64)k * nb + i];
+    const double t = block_sum(acc);
+    if (threadIdx.x == 0) dres[k] = t;
+}
+
+// out[c * R + r] = in[r * Cc + c]  (R x Cc -> Cc x R)
+__global__ __launch_bounds__(256) void k_transpose(const double *__restrict__ in, double *__restrict__ out,
+                                                   i64 R, i64 Cc) {
+    __shared__ double tile[32][33];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const i64 c0 = (i64)blockIdx.x * 32, r0 = (i64)blockIdx.y * 32;
+    for (int i = 0; i < 4; ++i) {
+        const i64 rr = r0 + ty + 8 * i, cc = c0 + tx;
+        if (rr < R && cc < Cc) tile[ty + 8 * i][tx] = in[rr * Cc + cc];
+    }
+    __syncthreads();
+    for (int i = 0; i < 4; ++i) {
+        const i64 cc = c0 + ty + 8 * i, rr = r0 + tx;
+        if (rr < R && cc < Cc) out[cc * R + rr] = tile[tx][ty + 8 * i];
+    }
+}
+
+struct Segment {
+    int kind, first, count, pa;
+};
+
+static size_t seg_lds_budget() {
+    int dev = 0, v = 0;
+    HIPOTF_CHECK(hipGetDevice(&dev));
+    HIPOTF_CHECK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    return std::min<size_t>((size_t)v, 160 * 1024) - 2048;
+}
+
+// group the sweep (generator indices in order) into fusable segments; nvec rows per element
+// (1 for the forward expm sweep, 2 for the adjoint sweep) decide what fits LDS
+static std::vector<Segment> plan_sweep(DevPool *P, const i64 *idxs, i64 n, int nvec) {
+    const DevBasis *B = P->basis;
+    const i64 NA = B->pA->N, NB = B->pB->N;
+    const size_t budget = seg_lds_budget();
+    const bool rowA_ok = (size_t)NB * nvec * 8 <= budget;
+    const bool rowB_ok = (size_t)NA * nvec * 8 <= budget;
+    const bool pair_ok = (size_t)NB * 2 * 2 * 8 <= budget;  // kernel reserves 4 rows
+    std::vector<Segment> segs;
+    for (i64 i = 0; i < n;) {
+        const Generator &G = P->gens[idxs[i]];
+        Segment s{SEG_SINGLE, (int)i, 1, G.pa};
+        auto gen = [&](i64 j) -> const Generator & { return P->gens[idxs[j]]; };
+        if (G.kind == 2 && rowA_ok) {
+            s.kind = SEG_ROW_A;
+            while (i + s.count < n && gen(i + s.count).kind == 2) ++s.count;
+        } else if ((G.kind == 3 || (G.kind == 1 && i + 1 < n && gen(i + 1).kind == 3 && gen(i + 1).pa == G.pa)) &&
+                   pair_ok) {
+            s.kind = SEG_PAIR;
+            while (i + s.count < n && gen(i + s.count).pa == G.pa && gen(i + s.count).kind != 2) ++s.count;
+        } else if (G.kind == 1 && rowB_ok) {
+            s.kind = SEG_ROW_B;
+            while (i + s.count < n && gen(i + s.count).kind == 1 &&
+                   !(i + s.count + 1 < n && gen(i + s.count + 1).kind == 3 &&
+                     gen(i + s.count + 1).pa == gen(i + s.count).pa))
+                ++s.count;
+        }
+        if (s.kind != SEG_SINGLE && s.count > 4096) s.count = 4096;
+        segs.push_back(s);
+        i += s.count;
+    }
+    return segs;
+}
+
+// Forward expm sweep then adjoint sweep, both fused where possible.  psi / lam device vectors;
+// dres[i] receives the raw backgrad value of sweep position i.
+void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, double *psi, double *lam,
+                  bool forward, bool backward, double *dres, hipStream_t st) {
+    DevBasis *B = P->basis;
+    const i64 NA = B->pA->N, NB = B->pB->N, dim = NA * NB;
+    for (i64 i = 0; i < n; ++i)
+        HIPOTF_REQUIRE(idxs[i] >= 0 && idxs[i] < (i64)P->gens.size(), "generator index out of range");
+    // descriptors of every sweep position (for ROW_B the list is the alpha pattern)
+    std::vector<SegGen> host(n);
+    auto fill = [&](const std::vector<Segment> &segs) {
+        for (const Segment &s : segs)
+            for (int k = 0; k < s.count; ++k) {
+                const i64 i = s.first + k;
+                const Generator &G = P->gens[idxs[i]];
+                SegGen &d = host[i];
+                d.g.nops = G.nops;
+                for (int q = 0; q < 8; ++q) {
+                    d.g.pos[q] = q < G.nops ? G.pos[q] : 0;
+                    d.g.cre[q] = q < G.nops ? G.cre[q] : 0;
+                }
+                const PairList &L = (s.kind == SEG_ROW_B) ? P->lists[G.pa] : P->lists[G.pb];
+                d.lst = L.uw.as<int32_t>();
+                d.len = L.len;
+                d.c = std::cos(thetas[i]);
+                d.s = std::sin(thetas[i]);
+            }
+    };
+    auto run = [&](const std::vector<Segment> &segs, bool back) {
+        fill(segs);
+        P->segdesc.ensure((size_t)std::max<i64>(n, 1) * sizeof(SegGen));
+        if (n) HIPOTF_CHECK(hipMemcpyAsync(P->segdesc.p, host.data(), n * sizeof(SegGen), hipMemcpyHostToDevice, st));
+        const SegGen *dg = P->segdesc.as<SegGen>();
+        const int nvec = back ? 2 : 1;
+        for (size_t si = 0; si < segs.size(); ++si) {
+            const Segment &s = back ? segs[segs.size() - 1 - si] : segs[si];
+            if (s.kind == SEG_SINGLE) {
+                for (int kk = 0; kk < s.count; ++kk) {
+                    const int k = back ? s.count - 1 - kk : kk;
+                    const i64 i = s.first + k;
+                    if (back) rot_backgrad(P, idxs[i], thetas[i], psi, lam, dres + i, st);
+                    else rot_expm(P, idxs[i], thetas[i], psi, st);
+                }
+                continue;
+            }
+            if (s.kind == SEG_ROW_A || s.kind == SEG_ROW_B) {
+                const bool A = s.kind == SEG_ROW_A;
+                const i64 nrows = A ? NA : NB, nel = A ? NB : NA;
+                double *l = psi, *r = lam;
+                if (!A) {
+                    P->tl.ensure(dim * sizeof(double));
+                    P->tr.ensure(dim * sizeof(double));
+                    dim3 g((unsigned)((NB + 31) / 32), (unsigned)((NA + 31) / 32));
+                    hipLaunchKernelGGL(k_transpose, g, dim3(256), 0, st, psi, P->tl.as<double>(), NA, NB);
+                    if (back) hipLaunchKernelGGL(k_transpose, g, dim3(256), 0, st, lam, P->tr.as<double>(), NA, NB);
+                    l = P->tl.as<double>();
+                    r = P->tr.as<double>();
+                }
+                const size_t smem = (size_t)nel * nvec * sizeof(double);
+                double *part = nullptr;
+                if (back) {
+                    P->segpart.ensure((size_t)s.count * nrows * sizeof(double));
+                    part = P->segpart.as<double>();
+                }
+                const u64 *rstr = A ? B->pA->strs.as<u64>() : B->pB->strs.as<u64>();
+                const u64 *estr = A ? B->pB->strs.as<u64>() : B->pA->strs.as<u64>();
+                if (back) {
+                    HIPOTF_CHECK(hipFuncSetAttribute((const void *)k_seg_row<true>,
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+                    hipLaunchKernelGGL(k_seg_row<true>, dim3((unsigned)nrows), dim3(256), smem, st, l, r, nrows,
+                                       nel, rstr, estr, A ? 1 : 0, dg + s.first, s.count, part);
+                } else {
+                    HIPOTF_CHECK(hipFuncSetAttribute((const void *)k_seg_row<false>,
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+                    hipLaunchKernelGGL(k_seg_row<false>, dim3((unsigned)nrows), dim3(256), smem, st, l,
+                                       (double *)nullptr, nrows, nel, rstr, estr, A ? 1 : 0, dg + s.first,
+                                       s.count, (double *)nullptr);
+                }
+                HIPOTF_CHECK(hipGetLastError());
+                if (!A) {
+                    dim3 g((unsigned)((NA + 31) / 32), (unsigned)((NB + 31) / 32));
+                    hipLaunchKernelGGL(k_transpose, g, dim3(256), 0, st, P->tl.as<double>(), psi, NB, NA);
+                    if (back) hipLaunchKernelGGL(k_transpose, g, dim3(256), 0, st, P->tr.as<double>(), lam, NB, NA);
+                }
+                if (back)
+                    hipLaunchKernelGGL(k_seg_reduce, dim3(s.count), dim3(256), 0, st, part, nrows, dres + s.first);
+                HIPOTF_CHECK(hipGetLastError());
+            } else {  // SEG_PAIR
+                const PairList &LA = P->lists[s.pa];
+                const i64 np = LA.len;
+                if (np == 0) {
+                    if (back) HIPOTF_CHECK(hipMemsetAsync(dres + s.first, 0, s.count * sizeof(double), st));
+                    continue;
+                }
+                const size_t smem = (size_t)NB * 4 * sizeof(double);
+                double *part = nullptr;
+                if (back) {
+                    P->segpart.ensure((size_t)s.count * np * sizeof(double));
+                    part = P->segpart.as<double>();
+                    HIPOTF_CHECK(hipFuncSetAttribute((const void *)k_seg_pair<true>,
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+                    hipLaunchKernelGGL(k_seg_pair<true>, dim3((unsigned)np), dim3(256), smem, st, psi, lam,
+                                       LA.uw.as<int32_t>(), np, NB, B->pA->strs.as<u64>(),
+                                       B->pB->strs.as<u64>(), dg + s.first, s.count, part);
+                } else {
+                    HIPOTF_CHECK(hipFuncSetAttribute((const void *)k_seg_pair<false>,
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+                    hipLaunchKernelGGL(k_seg_pair<false>, dim3((unsigned)np), dim3(256), smem, st, psi,
+                                       (double *)nullptr, LA.uw.as<int32_t>(), np, NB, B->pA->strs.as<u64>(),
+                                       B->pB->strs.as<u64>(), dg + s.first, s.count, (double *)nullptr);
+                }
+                HIPOTF_CHECK(hipGetLastError());
+                if (back)
+                    hipLaunchKernelGGL(k_seg_reduce, dim3(s.count), dim3(256), 0, st, part, np, dres + s.first);
+                HIPOTF_CHECK(hipGetLastError());
+            }
+        }
+    };
+    if (forward) run(plan_sweep(P, idxs, n, 1), false);
+    if (backward) run(plan_sweep(P, idxs, n, 2), true);
+}
+
+// ---------------------------------------------------------------------------------------
 // dot product (two-stage, deterministic)
 // ---------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_dot_partial(const double *__restrict__ a,

@@ -23,6 +23,12 @@ struct DstPack { double *p[MAXNV]; };
 
 __device__ __forceinline__ int parity64(u64 v) { return __popcll(v) & 1; }
 
+// v with its sign bit xor-ed with bit 31 of s (s is 0 or 0x80000000)
+__device__ __forceinline__ double flip_sign(double v, u32 s) {
+    return __longlong_as_double(__double_as_longlong(v) ^ ((long long)s << 32));
+}
+__device__ __forceinline__ float flip_sign(float v, u32 s) { return __uint_as_float(__float_as_uint(v) ^ s); }
+
 // ---------------------------------------------------------------------------------------
 // K1: gauge + dual layout (32x32 tiles through LDS)
 // ---------------------------------------------------------------------------------------
@@ -240,11 +246,16 @@ __global__ __launch_bounds__(1024) void k_ab_reg(const T *__restrict__ xt, T *__
     const bool live = Ib < NB;
     const u64 astr = strsA[Ia];
 
+    // lane-ready descriptors in LDS byte offsets: bits 0-17 the source element x~[Ja, J] inside
+    // the staged row (whole frame < 160 KB < 2^18), bits 18-30 the integral W[r*n+s]
+    // (n <= 32: < 8 KB), bit 31 the sign of <Ib|E_rs|J>
+    const u32 xoff = (u32)(((n2 + n + 15) & ~15) * sizeof(T));
     u32 d[SXMAX];
 #pragma unroll
     for (int c = 0; c < SXMAX; ++c) {
         u32 e = (live && c < nsxB) ? sxB[(i64)c * NB + Ib] : 0u;
-        d[c] = sx_J(e) | (u32(sx_r(e) * n + sx_s(e)) << 20) | (e & 0x80000000u);
+        d[c] = (xoff + sx_J(e) * (u32)(NVS * sizeof(T))) |
+               ((u32)((sx_r(e) * n + sx_s(e)) * sizeof(T)) << 18) | (e & 0x80000000u);
     }
     const u64 bocc = live ? strsB[Ib] : 0ull;
 
@@ -293,14 +304,14 @@ __global__ __launch_bounds__(1024) void k_ab_reg(const T *__restrict__ xt, T *__
 #pragma unroll
         for (int c = 0; c < SXMAX; ++c) {
             if (c < nsxB) {
-                // lane-ready descriptor: J (20 bits) | r*n+s (10 bits) << 20 | sign << 31.
-                // The empty asm keeps the decode inside the loop: hoisting it would cost two
-                // VGPRs per slot and spill.
+                // The empty asm ties the decode to this alpha connection (input ca), so it is
+                // not hoisted out of the ca loop (two VGPRs per slot would spill); it has no
+                // side effects, so the LDS loads still schedule freely around it.
                 u32 e = d[c];
-                asm volatile("" : "+v"(e));
-                T w = W[(e >> 20) & 0x3FFu];
-                if (e >> 31) w = -w;
-                const T *xp = xrow + (e & 0xFFFFFu) * NVS;
+                asm("" : "+v"(e) : "s"(ca));
+                const T w = flip_sign(*reinterpret_cast<const T *>(smem_raw + ((e >> 18) & 0x1FFFu)),
+                                      e & 0x80000000u);
+                const T *xp = reinterpret_cast<const T *>(smem_raw + (e & 0x3FFFFu));
 #pragma unroll
                 for (int v = 0; v < NVS; ++v) a[v] += w * xp[v];
             }

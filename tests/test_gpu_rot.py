@@ -57,9 +57,11 @@ def test_expm_sweep_and_backgrad(torch_cuda, n, na, nb, network_pool):
 
 @pytest.mark.parametrize("n,na,nb", CASES + [(12, 6, 6)])
 def test_fused_vqe_energy_grad(torch_cuda, n, na, nb):
+    """Full sweeps for n <= 8 exercise every fused segment kind (pure-beta rows, alpha-pattern
+    row pairs, pure-alpha rows via the beta-major copy); at 12 orbitals the first 60."""
     S = oracle_system(n, na, nb, pool=True)
     B, H, P, F, sos = _dev(n, na, nb)
-    npar = min(len(sos), 60)
+    npar = len(sos) if n <= 8 else 60
     idxs = np.arange(npar)
     th = np.random.default_rng(7).uniform(-0.05, 0.05, npar)
     lv, rv = O.hf_vector(S), np.zeros(S.dim)
@@ -71,6 +73,24 @@ def test_fused_vqe_energy_grad(torch_cuda, n, na, nb):
     assert abs(d2 - d2_ref) < 1e-10
     assert np.max(np.abs(g - g_ref)) < 1e-12
     assert np.max(np.abs(host(psi) - lv)) < 1e-12  # reverse sweep uncomputes the state
+
+
+def test_vqe_config2_matches_reference_fixture(torch_cuda):
+    """12 orbitals, 6+6 electrons, all 1,818 UCC parameters, theta ~ U[-0.05, 0.05]: the fused
+    energy + adjoint gradient equal the values the compiled reference produced (golden)."""
+    import os
+    f = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config0_n12.npz"))
+    B, H, P, F, sos = _dev(12, 6, 6)
+    th = f["vqe_thetas"]
+    assert len(th) == len(sos) == 1818
+    psi = torch_cuda.zeros(B.dim, dtype=torch_cuda.float64, device="cuda")
+    psi[0] = 1.0
+    lam = torch_cuda.zeros_like(psi)
+    e, g, d2, nrm = F.vqe_energy_grad(np.arange(len(th)), th, psi, lam)
+    assert abs(e - float(f["vqe_energy"])) < 1e-11
+    assert abs(d2 - float(f["vqe_d2h"])) < 1e-9
+    assert np.max(np.abs(g - f["vqe_grad"])) < 1e-11
+    assert abs(float(host(psi)[0]) - 1.0) < 1e-12 and abs(nrm - 1.0) < 1e-12
 
 
 def test_batchgrad(torch_cuda):
