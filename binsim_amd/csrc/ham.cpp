@@ -156,8 +156,12 @@ DevHam *make_dev_ham(DevBasis *b, int n, const std::vector<double> &h, const std
     H->ecore = ecore;
     H->h = h;
     H->eri = eri;
-    H->eri64.upload(H->eri);
-    std::vector<float> e32(H->eri.begin(), H->eri.end());
+    // ERI_PAD zero entries after the table: kernels that fill fixed-size LDS tiles from one
+    // (pq|..) row may read up to that far past the last row
+    std::vector<double> e64(H->eri);
+    e64.resize(H->eri.size() + ERI_PAD, 0.0);
+    H->eri64.upload(e64);
+    std::vector<float> e32(e64.begin(), e64.end());
     H->eri32.upload(e32);
     const HostBasis &hb = b->host;
     build_same_spin(H->tA, hb.A, H->h, H->eri, n);

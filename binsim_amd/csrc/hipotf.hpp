@@ -147,6 +147,8 @@ struct SameSpinTable {
     DevBuf gdiag;             // double[N]  same-spin diagonal G[I,I]
 };
 
+constexpr int ERI_PAD = 256;  // zero entries appended to the device integral tables
+
 struct DevHam {
     DevBasis *basis = nullptr;
     int n = 0;
