@@ -1,0 +1,290 @@
+// Alpha-beta part of the sigma build, source-lane form (included by sigma.hip).
+//
+// Workgroup = destination alpha row Ia; every thread owns one SOURCE beta string Jb of the
+// current chunk of 512 and a share (1/SPLIT) of its k*(n-k) single-excitation slots:
+//
+//   sigma~[Ia, Ib] += sum_{alpha conn. (Ja, pq, s_a)} sum_{slots (r in Jb, t not in Jb)}
+//                     s_a (pq|rt) <Ib|E_tr|Jb> x~[Ja, Jb]          Ib = Jb - r + t
+//                   + sum_{alpha conn.} (sum_{r in Jb} s_a (pq|rr)) x~[Ja, Jb]   (Ib = Jb)
+//
+// (the alpha connections are the nsxA single excitations Ja = Ia - p + q of Ia and Ia itself
+// with W = sum_{p in Ia} (pp|..)).  The source element x~[Ja, Jb] is loaded once per alpha
+// connection -- coalesced along the row, XPF connections ahead -- and reused from a register
+// for all of the thread's slots; the accumulators acc[slot] stay in registers across the
+// alpha connections and are scatter-added into an LDS copy of the destination row once per
+// chunk.  Slot (io, iv) is orbital occ[io] annihilated / vir[iv] created, the order of the
+// single-excitation table, so the scatter reads (Ib, sign) straight from that table.  By
+// Hermiticity <Ib|E_tr|Jb> = <Jb|E_rt|Ib>, the table's sign for Jb, and (pq|rt) = (pq|tr).
+//
+// The integral row W[r*n + t] = s_a (pq|rt) is stored R times interleaved (entry e, copy c at
+// e*R + c) and thread l reads copy l % R, so the lane-varying gathers W[occ*n + vir] of a
+// group of R lanes never share a bank.  W of the next connection is prefetched into
+// registers while the current one is consumed.  Contributions are added into the LDS row
+// one wave at a time, in wave order, so the sum order -- and the FP result -- is
+// deterministic.  Fixed block size, so fill positions are compile-time offsets.
+// Included inside namespace hipotf after sx_* and u32/u64/i64 are defined.
+#pragma once
+
+constexpr int AB_SRC_NJ = 512;   // source strings per chunk
+constexpr int AB_SRC_NWE = 256;  // integral entries one W fill covers (n*n <= 256)
+constexpr int AB_SRC_XPF = 3;    // alpha connections of source elements in flight
+
+template <typename T, int NVS, int KM, int VM, int R, int SPLIT>
+__global__ __launch_bounds__(AB_SRC_NJ *SPLIT) __attribute__((amdgpu_waves_per_eu(2 * SPLIT, 2 * SPLIT)))
+void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, const u32 *__restrict__ sxA,
+              const u32 *__restrict__ sxB, const u64 *__restrict__ strsA, const u64 *__restrict__ strsB,
+              const T *__restrict__ eri, int n, i64 NA, i64 NB, int nsxA, int kB, int vB) {
+    constexpr int BD = AB_SRC_NJ * SPLIT;
+    constexpr int KH = KM / SPLIT;                 // occupied-orbital slots per thread
+    constexpr int NFM = AB_SRC_NWE * R / BD;       // W values each thread fills
+    constexpr int XPF = AB_SRC_XPF;
+    static_assert(KM % SPLIT == 0 && (AB_SRC_NWE * R) % BD == 0 && BD % R == 0, "fill layout");
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int n2 = n * n;
+    T *Wr = reinterpret_cast<T *>(smem_raw);       // AB_SRC_NWE * R replicated integral row
+    T *Wdg = Wr + AB_SRC_NWE * R;                  // AB_SRC_NWE + 16: sum_{p in Ia} (pp|rt), zero padded
+    T *S = Wdg + AB_SRC_NWE + 16;                  // NB * NVS destination row
+    constexpr int BW = KM + 2;
+    int *binom = reinterpret_cast<int *>(S + NB * NVS);  // C(p, j), p < n, j < BW
+    const int tid = threadIdx.x;
+    const int jl = tid & (AB_SRC_NJ - 1);          // source string within the chunk
+    const int half = SPLIT == 1 ? 0 : __builtin_amdgcn_readfirstlane(tid / AB_SRC_NJ);
+    const i64 per = (NA + 7) / 8;
+    const i64 Ia = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (Ia >= NA) return;
+    const u64 astr = strsA[Ia];
+    for (i64 t = tid; t < NB * NVS; t += BD) S[t] = T(0);
+    for (int t = tid; t < AB_SRC_NWE + 16; t += BD) {
+        T v = 0;
+        u64 m = t < n2 ? astr : 0ull;
+        while (m) {
+            const int p = __builtin_ctzll(m);
+            m &= m - 1;
+            v += eri[(i64)(p * n + p) * n2 + t];
+        }
+        Wdg[t] = v;
+    }
+    for (int t = tid; t < n * BW; t += BD) {
+        const int p = t / BW, j = t - p * BW;
+        u64 c = j <= p ? 1ull : 0ull;
+        for (int i = 0; i < j && c; ++i) c = c * (u64)(p - i) / (u64)(i + 1);
+        binom[t] = (int)c;
+    }
+    __syncthreads();
+
+    const int nca = nsxA + 1;
+    const i64 nchunks = (NB + AB_SRC_NJ - 1) / AB_SRC_NJ;
+    const i64 nit = nchunks * nca;
+    const u32 copy_off = (u32)((tid & (R - 1)) * sizeof(T));
+    const int e0 = tid / R;  // this thread fills entries e0 + f * (BD / R)
+
+    // W values of connection ca (the sign is applied when they are stored, so the loads stay
+    // in flight through the compute phase).  Entries past n2 run into the next row or the
+    // zero padding after the table and land in Wr slots no lane reads.
+    auto load_w = [&](T (&pre)[NFM], bool &neg, int ca) {
+        if (ca < nsxA) {
+            const u32 e = sxA[(i64)ca * NA + Ia];
+            const T *er = eri + (i64)(sx_r(e) * n + sx_s(e)) * n2 + e0;
+            neg = sx_neg(e);
+#pragma unroll
+            for (int f = 0; f < NFM; ++f) pre[f] = er[f * (BD / R)];
+        } else {
+            neg = false;
+#pragma unroll
+            for (int f = 0; f < NFM; ++f) pre[f] = Wdg[e0 + f * (BD / R)];
+        }
+    };
+    // source elements of the round (chunk, connection ca); rounds are tracked by incremental
+    // counters: a 64-bit division here would be expanded into vector code in the FMA phase
+    auto load_x = [&](T (&xv)[NVS], i64 chunk, int ca) {
+        if (chunk >= nchunks) return;
+        const i64 Jb = chunk * AB_SRC_NJ + jl;
+        const i64 Ja = ca < nsxA ? (i64)sx_J(sxA[(i64)ca * NA + Ia]) : Ia;
+#pragma unroll
+        for (int v = 0; v < NVS; ++v) xv[v] = Jb < NB ? xt[(Ja * NB + Jb) * nvst + v0 + v] : T(0);
+    };
+
+    T pre[NFM];
+    bool pneg = false;
+    load_w(pre, pneg, 0);
+    T xq[XPF][NVS];
+    i64 pchunk = 0;  // round whose source elements are loaded next
+    int pca = 0;
+    auto advance = [&](i64 &c, int &a) {
+        if (++a == nca) {
+            a = 0;
+            ++c;
+        }
+    };
+#pragma unroll
+    for (int d = 0; d < XPF; ++d) {
+#pragma unroll
+        for (int v = 0; v < NVS; ++v) xq[d][v] = T(0);
+        load_x(xq[d], pchunk, pca);
+        advance(pchunk, pca);
+    }
+    T acc[KH][VM][NVS], accd[NVS];
+    u32 oo[KH], vv[VM], dd[KH];
+    i64 Jb = 0;
+    bool live = false;
+    // slots beyond (kB, vB) of the instantiated KM x VM grid read valid LDS words into
+    // accumulators that are never scattered; their diagonal terms read the zero padding
+    const u32 zero_off = (u32)((AB_SRC_NWE * R + AB_SRC_NWE) * sizeof(T));
+    i64 chunk = 0;
+    int ca = 0;
+    for (i64 it = 0; it < nit; ++it, advance(chunk, ca)) {
+        if (ca == 0) {
+            Jb = chunk * AB_SRC_NJ + jl;
+            live = Jb < NB;
+            const u64 s = live ? strsB[Jb] : 0ull;
+            u64 mo = s, mv = ~s & ((n < 64) ? ((1ull << n) - 1) : ~0ull);
+            for (int i = 0; i < half * KH; ++i) mo &= mo - 1;  // skip the other shares
+#pragma unroll
+            for (int i = 0; i < KH; ++i) {
+                const u32 p = mo ? (u32)__builtin_ctzll(mo) : 0u;
+                mo &= mo - 1;
+                oo[i] = p * (u32)(n * R * sizeof(T));
+                dd[i] = half * KH + i < kB ? p * (u32)((n + 1) * R * sizeof(T)) + copy_off : zero_off;
+            }
+#pragma unroll
+            for (int iv = 0; iv < VM; ++iv) {
+                const u32 q = mv ? (u32)__builtin_ctzll(mv) : 0u;
+                mv &= mv - 1;
+                vv[iv] = q * (u32)(R * sizeof(T)) + copy_off;
+            }
+#pragma unroll
+            for (int i = 0; i < KH; ++i)
+#pragma unroll
+                for (int iv = 0; iv < VM; ++iv)
+#pragma unroll
+                    for (int v = 0; v < NVS; ++v) acc[i][iv][v] = T(0);
+#pragma unroll
+            for (int v = 0; v < NVS; ++v) accd[v] = T(0);
+        }
+        __syncthreads();  // everyone is done reading the previous W
+#pragma unroll
+        for (int f = 0; f < NFM; ++f) Wr[tid + f * BD] = pneg ? -pre[f] : pre[f];
+        __syncthreads();
+        if (it + 1 < nit) load_w(pre, pneg, ca + 1 == nca ? 0 : ca + 1);
+        T xnew[NVS];
+#pragma unroll
+        for (int v = 0; v < NVS; ++v) xnew[v] = T(0);
+        load_x(xnew, pchunk, pca);
+        advance(pchunk, pca);
+        {
+            // the empty asm makes the row offsets opaque per iteration, so the KH*VM slot
+            // addresses are formed in the loop (one add each) instead of hoisted into VGPRs
+            u32 ob[KH];
+#pragma unroll
+            for (int i = 0; i < KH; ++i) {
+                ob[i] = oo[i];
+                asm("" : "+v"(ob[i]) : "s"(ca));
+            }
+            // one row of VM gathers in flight before its FMAs; the scheduling barrier keeps
+            // the compiler from overlapping rows (more gathers in flight would push the
+            // accumulators out of registers)
+#pragma unroll
+            for (int i = 0; i < KH; ++i) {
+                T wv[VM];
+#pragma unroll
+                for (int iv = 0; iv < VM; ++iv) wv[iv] = *reinterpret_cast<const T *>(smem_raw + ob[i] + vv[iv]);
+#pragma unroll
+                for (int iv = 0; iv < VM; ++iv)
+#pragma unroll
+                    for (int v = 0; v < NVS; ++v) acc[i][iv][v] += wv[iv] * xq[0][v];
+            }
+            T wd = 0;
+#pragma unroll
+            for (int i = 0; i < KH; ++i) wd += *reinterpret_cast<const T *>(smem_raw + dd[i]);
+#pragma unroll
+            for (int v = 0; v < NVS; ++v) accd[v] += wd * xq[0][v];
+        }
+        // advance the source-element queue
+#pragma unroll
+        for (int d = 0; d + 1 < XPF; ++d)
+#pragma unroll
+            for (int v = 0; v < NVS; ++v) xq[d][v] = xq[d + 1][v];
+#pragma unroll
+        for (int v = 0; v < NVS; ++v) xq[XPF - 1][v] = xnew[v];
+        if (ca == nca - 1) {
+            // scatter this chunk's contributions, one wave after the other.  The target of
+            // slot (io, iv) is computed, not loaded: with p_0 < .. < p_{k-1} the occupied and
+            // t_iv the virtual orbitals of Jb, m = t - iv occupied orbitals lie below t, and
+            // in colex order (rank = sum_j C(p_j, j+1))
+            //   t > r = p_io:  rank(Jb - r + t) = Jb - C(r, io+1) + C(t, m)   + sum_{io<j<m}  [C(p_j, j)   - C(p_j, j+1)]
+            //   t < r:         rank(Jb - r + t) = Jb - C(r, io+1) + C(t, m+1) + sum_{m<=j<io} [C(p_j, j+2) - C(p_j, j+1)]
+            // split into a per-io and a per-iv term; the sign is the parity of the occupied
+            // orbitals strictly between r and t: m - io - 1 (t > r) or io - m (t < r).
+            constexpr int nwaves = BD / 64;
+            const int wave = tid >> 6;
+            for (int w = 0; w < nwaves; ++w) {
+                if (wave == w && live) {
+                    // (computed inside the turn so the temporaries are short-lived)
+                    const u64 s = strsB[Jb];
+                    // occupied orbitals and, per own occupied slot, -C(r,io+1) - A_{io+1} and
+                    // -C(r,io+1) + B_io with A_j = sum_{i<j} C(p_i,i)-C(p_i,i+1),
+                    // B_j = sum_{i<j} C(p_i,i+2)-C(p_i,i+1)
+                    int pa[KM], up[KH], dn[KH];
+                    {
+                        u64 mo = s;
+                        int A = 0, Bp = 0;
+#pragma unroll
+                        for (int j = 0; j < KM; ++j) {
+                            const int p = mo ? __builtin_ctzll(mo) : 0;
+                            mo &= mo - 1;
+                            pa[j] = p;
+                            const int c0 = binom[p * BW + j], c1 = binom[p * BW + j + 1], c2 = binom[p * BW + j + 2];
+                            const int a = j < kB ? c0 - c1 : 0, b = j < kB ? c2 - c1 : 0;
+                            const int i = j - half * KH;
+                            if (i >= 0 && i < KH) {
+                                up[i] = -c1 - (A + a);
+                                dn[i] = -c1 + Bp;
+                            }
+                            A += a;
+                            Bp += b;
+                        }
+                    }
+                    u64 mv = ~s & ((n < 64) ? ((1ull << n) - 1) : ~0ull);
+#pragma unroll
+                    for (int iv = 0; iv < VM; ++iv) {
+                        const int t = mv ? __builtin_ctzll(mv) : 0;
+                        mv &= mv - 1;
+                        if (iv < vB) {
+                            // C(t, m) + A_m and C(t, m+1) - B_m
+                            const int m = t - iv;
+                            int F = binom[t * BW + m], G = binom[t * BW + m + 1];
+#pragma unroll
+                            for (int j = 0; j < KM; ++j) {
+                                if (j < kB && pa[j] < t) {
+                                    const int c1 = binom[pa[j] * BW + j + 1];
+                                    F += binom[pa[j] * BW + j] - c1;
+                                    G -= binom[pa[j] * BW + j + 2] - c1;
+                                }
+                            }
+#pragma unroll
+                            for (int i = 0; i < KH; ++i) {
+                                const int io = half * KH + i;
+                                if (io < kB) {
+                                    const bool above = m > io;
+                                    const i64 Ib = Jb + (above ? up[i] + F : dn[i] + G);
+                                    const bool neg = (m + io + (above ? 1 : 0)) & 1;
+#pragma unroll
+                                    for (int v = 0; v < NVS; ++v)
+                                        atomicAdd(&S[Ib * NVS + v], neg ? -acc[i][iv][v] : acc[i][iv][v]);
+                                }
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int v = 0; v < NVS; ++v) atomicAdd(&S[Jb * NVS + v], accd[v]);
+                }
+                __syncthreads();
+            }
+        }
+    }
+    __syncthreads();
+    for (i64 t = tid; t < NB * NVS; t += BD) {
+        const i64 Ib = t / NVS;
+        s1[(Ia * NB + Ib) * nvst + v0 + (t - Ib * NVS)] += S[t];
+    }
+}

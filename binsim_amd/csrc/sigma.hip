@@ -350,215 +350,7 @@ __global__ __launch_bounds__(1024) void k_ab_reg(const T *__restrict__ xt, T *__
     }
 }
 
-// ---------------------------------------------------------------------------------------
-// K4 (source-lane form): workgroup = destination alpha row Ia, lane = SOURCE beta string Jb.
-//
-//   sigma~[Ia, Ib] += sum_{alpha conn. (Ja, pq, s_a)} sum_{slots (r in Jb, t not in Jb)}
-//                     s_a (pq|rt) <Ib|E_tr|Jb> x~[Ja, Jb]          Ib = Jb - r + t
-//                   + sum_{alpha conn.} (sum_{r in Jb} s_a (pq|rr)) x~[Ja, Jb]   (Ib = Jb)
-//
-// The source element x~[Ja, Jb] is loaded once per alpha connection (coalesced along the
-// row) and reused from a register for all k*(n-k) beta slots of the lane; the accumulators
-// acc[slot] stay in registers across the alpha connections and are scatter-added into an
-// LDS copy of the destination row once per chunk of source strings.  Slot (io, iv) is
-// orbital occ[io] annihilated / vir[iv] created, the order of the single-excitation table,
-// so the scatter reads (Ib, sign) straight from that table.  By Hermiticity
-// <Ib|E_tr|Jb> = <Jb|E_rt|Ib>, the table's sign for Jb, and (pq|rt) = (pq|tr).
-//
-// The integral row W[r*n + t] = s_a (pq|rt) is stored R times interleaved (entry e, copy c
-// at e*R + c) and lane l reads copy l % R, so a group of R lanes never shares a bank: the
-// lane-varying gathers W[occ*n + vir] are conflict-free.  W for the next connection is
-// prefetched into registers (NFM values per thread) while the current one is consumed.
-// Contributions are added into the LDS row one wave at a time, in wave order, so the sum
-// order -- and the FP result -- is deterministic.  The block size is fixed (AB_SRC_BD) so
-// the fill positions are compile-time offsets.
-// ---------------------------------------------------------------------------------------
-constexpr int AB_SRC_BD = 512;
-
-template <typename T, int NVS, int KM, int VM, int R, int NFM>
-__global__ __launch_bounds__(AB_SRC_BD) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1,
-                                                      int nvst, int v0, const u32 *__restrict__ sxA,
-                                                      const u32 *__restrict__ sxB,
-                                                      const u64 *__restrict__ strsA,
-                                                      const u64 *__restrict__ strsB,
-                                                      const T *__restrict__ eri, int n, i64 NA, i64 NB,
-                                                      int nsxA, int kB, int vB) {
-    static_assert(AB_SRC_BD % R == 0, "block size must be a multiple of the replication");
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int n2 = n * n;
-    T *Wr = reinterpret_cast<T *>(smem_raw);        // n2 * R replicated integral row
-    constexpr int bd = AB_SRC_BD;
-    constexpr int NWE = NFM * bd / R;               // integral entries one fill covers (>= n2)
-    T *Wdg = Wr + NFM * bd;                         // NWE + 16: sum_{p in Ia} (pp|rt), zero padded
-    T *S = Wdg + NWE + 16;                          // NB * NVS destination row
-    const int tid = threadIdx.x;
-    const i64 per = (NA + 7) / 8;
-    const i64 Ia = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (Ia >= NA) return;
-    const u64 astr = strsA[Ia];
-    for (i64 t = tid; t < NB * NVS; t += bd) S[t] = T(0);
-    for (int t = tid; t < NWE + 16; t += bd) {
-        T v = 0;
-        u64 m = t < n2 ? astr : 0ull;
-        while (m) {
-            const int p = __builtin_ctzll(m);
-            m &= m - 1;
-            v += eri[(i64)(p * n + p) * n2 + t];
-        }
-        Wdg[t] = v;
-    }
-    __syncthreads();
-
-    const int nca = nsxA + 1;
-    const i64 nchunks = (NB + bd - 1) / bd;
-    const i64 nit = nchunks * nca;
-    const u32 copy_off = (u32)((tid & (R - 1)) * sizeof(T));
-
-    // W values of connection ca for this thread's fill positions tid + f*bd, i.e. entries
-    // tid/R + f*(bd/R)
-    const int e0 = tid / R;
-    // (the sign is applied when the values are stored, so the loads stay in flight)
-    auto load_w = [&](T (&pre)[NFM], bool &neg, int ca) {
-        if (ca < nsxA) {
-            const u32 e = sxA[(i64)ca * NA + Ia];
-            const T *er = eri + (i64)(sx_r(e) * n + sx_s(e)) * n2 + e0;
-            neg = sx_neg(e);
-            // entries past n2 run into the next row (or the zero padding after the table);
-            // they land in Wr slots no lane reads
-#pragma unroll
-            for (int f = 0; f < NFM; ++f) pre[f] = er[f * (bd / R)];
-        } else {
-            neg = false;
-#pragma unroll
-            for (int f = 0; f < NFM; ++f) pre[f] = Wdg[e0 + f * (bd / R)];
-        }
-    };
-    auto load_x = [&](T (&xv)[NVS], i64 chunk, int ca) {
-        const i64 Jb = chunk * bd + tid;
-        const i64 Ja = ca < nsxA ? (i64)sx_J(sxA[(i64)ca * NA + Ia]) : Ia;
-#pragma unroll
-        for (int v = 0; v < NVS; ++v) xv[v] = Jb < NB ? xt[(Ja * NB + Jb) * nvst + v0 + v] : T(0);
-    };
-
-    T pre[NFM];
-    bool pneg = false;
-    T xv[NVS], xn[NVS];
-    load_w(pre, pneg, 0);
-    load_x(xv, 0, 0);
-    T acc[KM][VM][NVS], accd[NVS];
-    u32 oo[KM], vv[VM], dd[KM];
-    i64 Jb = 0;
-    bool live = false;
-    // slots beyond (kB, vB) of the instantiated KM x VM grid read valid LDS words into
-    // accumulators that are never scattered; their diagonal terms read the zero padding
-    const u32 zero_off = (u32)((NFM * bd + NWE) * sizeof(T));
-    for (i64 it = 0; it < nit; ++it) {
-        const int ca = (int)(it % nca);
-        const i64 chunk = it / nca;
-        if (ca == 0) {
-            Jb = chunk * bd + tid;
-            live = Jb < NB;
-            const u64 s = live ? strsB[Jb] : 0ull;
-            u64 mo = s, mv = ~s & ((n < 64) ? ((1ull << n) - 1) : ~0ull);
-#pragma unroll
-            for (int io = 0; io < KM; ++io) {
-                const u32 p = mo ? (u32)__builtin_ctzll(mo) : 0u;
-                mo &= mo - 1;
-                oo[io] = p * (u32)(n * R * sizeof(T));
-                dd[io] = io < kB ? p * (u32)((n + 1) * R * sizeof(T)) + copy_off : zero_off;
-            }
-#pragma unroll
-            for (int iv = 0; iv < VM; ++iv) {
-                const u32 q = mv ? (u32)__builtin_ctzll(mv) : 0u;
-                mv &= mv - 1;
-                vv[iv] = q * (u32)(R * sizeof(T)) + copy_off;
-            }
-#pragma unroll
-            for (int io = 0; io < KM; ++io)
-#pragma unroll
-                for (int iv = 0; iv < VM; ++iv)
-#pragma unroll
-                    for (int v = 0; v < NVS; ++v) acc[io][iv][v] = T(0);
-#pragma unroll
-            for (int v = 0; v < NVS; ++v) accd[v] = T(0);
-        }
-        __syncthreads();  // everyone is done reading the previous W
-#pragma unroll
-        for (int f = 0; f < NFM; ++f) Wr[tid + f * bd] = pneg ? -pre[f] : pre[f];
-        __syncthreads();
-        if (it + 1 < nit) {
-            const int ca1 = ca + 1 == nca ? 0 : ca + 1;
-            load_w(pre, pneg, ca1);
-            load_x(xn, ca1 == 0 ? chunk + 1 : chunk, ca1);
-        }
-        {
-            // the empty asm makes the row offsets opaque per iteration, so the KM*VM slot
-            // addresses are formed in the loop (one add each) instead of hoisted into VGPRs
-            u32 ob[KM];
-#pragma unroll
-            for (int io = 0; io < KM; ++io) {
-                ob[io] = oo[io];
-                asm("" : "+v"(ob[io]) : "s"(ca));
-            }
-            // one row of VM gathers in flight before its FMAs
-#pragma unroll
-            for (int io = 0; io < KM; ++io) {
-                T wv[VM];
-#pragma unroll
-                for (int iv = 0; iv < VM; ++iv) wv[iv] = *reinterpret_cast<const T *>(smem_raw + ob[io] + vv[iv]);
-#pragma unroll
-                for (int iv = 0; iv < VM; ++iv)
-#pragma unroll
-                    for (int v = 0; v < NVS; ++v) acc[io][iv][v] += wv[iv] * xv[v];
-            }
-            T wd = 0;
-#pragma unroll
-            for (int io = 0; io < KM; ++io) wd += *reinterpret_cast<const T *>(smem_raw + dd[io]);
-#pragma unroll
-            for (int v = 0; v < NVS; ++v) accd[v] += wd * xv[v];
-        }
-#pragma unroll
-        for (int v = 0; v < NVS; ++v) xv[v] = xn[v];
-        if (ca == nca - 1) {
-            // scatter this chunk's contributions, one wave after the other
-            const int nwaves = bd >> 6, wave = tid >> 6;
-            for (int w = 0; w < nwaves; ++w) {
-                if (wave == w && live) {
-                    // an opaque copy of the table pointer keeps the KM*VM descriptor addresses
-                    // from being hoisted out of the w loop into VGPRs
-                    const u32 *sxJ = sxB + Jb;
-                    asm("" : "+v"(sxJ) : "s"(w));
-#pragma unroll
-                    for (int io = 0; io < KM; ++io) {
-                        // keeps the descriptor loads of one io group from being batched
-                        // with the next (register pressure)
-                        asm volatile("" ::: "memory");
-                        if (io < kB) {
-#pragma unroll
-                            for (int iv = 0; iv < VM; ++iv) {
-                                if (iv < vB) {
-                                    const u32 e = sxJ[(i64)(io * vB + iv) * NB];
-                                    const i64 Ib = sx_J(e);
-#pragma unroll
-                                    for (int v = 0; v < NVS; ++v)
-                                        atomicAdd(&S[Ib * NVS + v], sx_neg(e) ? -acc[io][iv][v] : acc[io][iv][v]);
-                                }
-                            }
-                        }
-                    }
-#pragma unroll
-                    for (int v = 0; v < NVS; ++v) atomicAdd(&S[Jb * NVS + v], accd[v]);
-                }
-                __syncthreads();
-            }
-        }
-    }
-    __syncthreads();
-    for (i64 t = tid; t < NB * NVS; t += bd) {
-        const i64 Ib = t / NVS;
-        s1[(Ia * NB + Ib) * nvst + v0 + (t - Ib * NVS)] += S[t];
-    }
-}
+#include "ab_src.hpp"
 
 // ---------------------------------------------------------------------------------------
 // K5: sigma = E (s1 + s2T^T + E_core x~), FP64 output vectors
@@ -682,18 +474,24 @@ static bool try_ab_src(DevHam *H, const T *xt, T *s1, int nvst, int v0, const T 
         const DevSpin &A = *B->pA, &Bs = *B->pB;
         const int n = H->n, n2 = n * n;
         const int kB = Bs.k, vB = Bs.n - Bs.k;
-        // LDS banks are 4 bytes wide and serve 128 bytes per clock: 16 lanes of 8-byte or 32
-        // lanes of 4-byte reads go together, so R copies make those groups conflict-free
+        // R copies of the integral row: 32 lanes of 4-byte reads are served together, so 32
+        // copies make them conflict-free; 8-byte reads get 16 (LDS space)
         constexpr int R = sizeof(T) == 8 ? 16 : 32;
-        constexpr int NFM = sizeof(T) == 8 ? 8 : 16;
+        // threads per source string: 1 (512-thread blocks, 2 waves per SIMD, accumulators in
+        // up to 256 VGPRs); HIPOTF_AB_SPLIT = 2 takes 1024-thread blocks with half the slots
+        // per thread (measurement)
+        static const int split_env = [] {
+            const char *s = std::getenv("HIPOTF_AB_SPLIT");
+            return s ? std::atoi(s) : 0;
+        }();
+        const int split = split_env == 2 ? 2 : 1;
         const i64 NB = Bs.N;
-        constexpr int bd = AB_SRC_BD;
-        constexpr int NWE = NFM * bd / R;
-        if (n2 > NWE) return false;
-        // a W fill reads NWE entries from the start of one (pq|..) row
-        static_assert(NWE <= ERI_PAD + 1, "integral tables are padded for one fill past the last row");
-        // the 8-byte FP64 accumulators of the destination row must stay 8-byte aligned
-        const size_t smem = ((size_t)NFM * bd + NWE + 16 + (size_t)NB * NVS) * sizeof(T);
+        const int bd = AB_SRC_NJ * split;
+        if (n2 > AB_SRC_NWE) return false;
+        // a W fill reads AB_SRC_NWE entries from the start of one (pq|..) row
+        static_assert(AB_SRC_NWE <= ERI_PAD + 1, "integral tables are padded for one fill past the last row");
+        // + the binomial table C(p, j), p < n <= 16, j < KM + 2 <= 10
+        const size_t smem = ((size_t)AB_SRC_NWE * R + AB_SRC_NWE + 16 + (size_t)NB * NVS) * sizeof(T) + 16 * 10 * 4;
         if (smem > std::min<size_t>(lds_budget(), 160 * 1024)) return false;
         const i64 per = (A.N + 7) / 8;
         const i64 nblocks = per * 8;
@@ -706,12 +504,16 @@ static bool try_ab_src(DevHam *H, const T *xt, T *s1, int nvst, int v0, const T 
                                n, A.N, NB, A.nsx, kB, vB);
             HIPOTF_CHECK(hipGetLastError());
         };
-        if (kB <= 4 && vB <= 4) go(k_ab_src<T, NVS, 4, 4, R, NFM>);
-        else if (kB <= 6 && vB <= 6) go(k_ab_src<T, NVS, 6, 6, R, NFM>);
-        else if constexpr (sizeof(T) == 8 && NVS == 2) return false;  // 128 FP64 accumulators: spills
-        else if (kB <= 8 && vB <= 8) go(k_ab_src<T, NVS, 8, 8, R, NFM>);
-        else return false;
-        return true;
+        auto pick = [&](auto split_c) {
+            constexpr int SPLIT = decltype(split_c)::value;
+            if (kB <= 4 && vB <= 4) go(k_ab_src<T, NVS, 4, 4, R, SPLIT>);
+            else if (kB <= 6 && vB <= 6) go(k_ab_src<T, NVS, 6, 6, R, SPLIT>);
+            else if constexpr (sizeof(T) == 8 && NVS == 2) return false;  // 128 FP64 accumulators: spills
+            else if (kB <= 8 && vB <= 8) go(k_ab_src<T, NVS, 8, 8, R, SPLIT>);
+            else return false;
+            return true;
+        };
+        return split == 2 ? pick(std::integral_constant<int, 2>{}) : pick(std::integral_constant<int, 1>{});
     }
 }
 
