@@ -33,6 +33,12 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
 
+# HBM bytes of one FP64 sigma build at 16 orbitals 8a8b, from two rocprofv3 --pmc runs of
+# `bench.py --pmc-unit` (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, summed over the five
+# kernels of one sigma), each scaled by what the same counter reads for a plain copy kernel of
+# 1.325 GB in the same run (profiles/sigma_fp64_16o_pmc.md)
+SIGMA_TRAFFIC_16O = None
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -179,6 +185,8 @@ def main():
     ap.add_argument("--full", action="store_true",
                     help="also run the 4-root FCI solves at 16 orbitals (plain and mixed; ~15 min)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pmc-unit", action="store_true",
+                    help="one FP64 sigma and one plain copy of the same size (counter runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -192,6 +200,12 @@ def main():
     setup_s = time.perf_counter() - t
     x = seeded(torch, B.dim, 1000 + 16)
     y = torch.empty_like(x)
+    if args.pmc_unit:
+        F.hvec(x, y)
+        torch.mul(x, 1.0, out=y)  # calibration: reads and writes 8 * dim bytes
+        torch.cuda.synchronize()
+        log(f"pmc unit done: one sigma, one copy of {8 * B.dim} bytes")
+        return
     ms, wall = timed_sigma(torch, F, x, y, args.steps, args.warmup)
     bs = sigma_bytes(n, na)
     gbs = bs / (ms * 1e-3) / 1e9
@@ -218,9 +232,11 @@ def main():
         "config": {"workload": "FCI sigma build H x, 16 orbitals 8a8b (165,636,900 determinants)",
                    "norb": n, "nalpha": na, "nbeta": nb, "dim": B.dim, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4),
+                     "traffic": SIGMA_TRAFFIC_16O,
                      "algorithmic_bytes_per_launch": bs,
-                     "kernel": "one hvec_cuda = 5 kernels (gauge, 2 x row-SpMM, alpha-beta, finalize)"},
+                     "kernel": "one sigma build (hvec_cuda): k_gauge_in, 2 x k_rowspmm, k_ab_src, "
+                               "k_finalize on the default stream, timed with HIP events as one unit"},
         "sigma_fp32_block4_ms_per_vector": round(ms32, 3),
         "setup_s": round(setup_s, 3), "wall_s_timed": round(wall, 3),
     }

@@ -33,7 +33,7 @@ template <typename T, int NVS, int KM, int VM, int R, int SPLIT>
 __global__ __launch_bounds__(AB_SRC_NJ *SPLIT) __attribute__((amdgpu_waves_per_eu(2 * SPLIT, 2 * SPLIT)))
 void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, const u32 *__restrict__ sxA,
               const u32 *__restrict__ sxB, const u64 *__restrict__ strsA, const u64 *__restrict__ strsB,
-              const T *__restrict__ eri, int n, i64 NA, i64 NB, int nsxA, int kB, int vB) {
+              const T *__restrict__ eri, int n, i64 NA, i64 NB, int nsxA, int kB, int vB, int ordered) {
     constexpr int BD = AB_SRC_NJ * SPLIT;
     constexpr int KH = KM / SPLIT;                 // occupied-orbital slots per thread
     constexpr int NFM = AB_SRC_NWE * R / BD;       // W values each thread fills
@@ -217,8 +217,11 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
             // orbitals strictly between r and t: m - io - 1 (t > r) or io - m (t < r).
             constexpr int nwaves = BD / 64;
             const int wave = tid >> 6;
-            for (int w = 0; w < nwaves; ++w) {
-                if (wave == w && live) {
+            // ordered: one wave after the other, so the LDS adds into each destination
+            // element happen in a fixed order (bitwise reproducible); otherwise all waves at
+            // once and the rounding of the sum depends on their timing
+            for (int w = 0; w < (ordered ? nwaves : 1); ++w) {
+                if ((!ordered || wave == w) && live) {
                     // (computed inside the turn so the temporaries are short-lived)
                     const u64 s = strsB[Jb];
                     // occupied orbitals and, per own occupied slot, -C(r,io+1) - A_{io+1} and

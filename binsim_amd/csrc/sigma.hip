@@ -465,10 +465,20 @@ static int ab_kernel_pref() {
     return pref;
 }
 
+// HIPOTF_DETERMINISTIC=1: bitwise reproducible sigma (the alpha-beta scatter runs wave by
+// wave, ~15% slower); by default its FP summation order depends on scheduling
+static bool deterministic() {
+    static const bool det = [] {
+        const char *s = std::getenv("HIPOTF_DETERMINISTIC");
+        return s && std::atoi(s) != 0;
+    }();
+    return det;
+}
+
 template <typename T, int NVS>
 static bool try_ab_src(DevHam *H, const T *xt, T *s1, int nvst, int v0, const T *eri, hipStream_t st) {
-    if constexpr (NVS > 2) {
-        return false;
+    if constexpr (NVS > 2 || (sizeof(T) == 8 && NVS == 2)) {
+        return false;  // accumulators would not fit the register file
     } else {
         DevBasis *B = H->basis;
         const DevSpin &A = *B->pA, &Bs = *B->pB;
@@ -501,7 +511,7 @@ static bool try_ab_src(DevHam *H, const T *xt, T *s1, int nvst, int v0, const T 
                                              (int)smem));
             hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(bd), smem, st, xt, s1, nvst, v0,
                                A.sx.as<u32>(), Bs.sx.as<u32>(), A.strs.as<u64>(), Bs.strs.as<u64>(), eri,
-                               n, A.N, NB, A.nsx, kB, vB);
+                               n, A.N, NB, A.nsx, kB, vB, deterministic() ? 1 : 0);
             HIPOTF_CHECK(hipGetLastError());
         };
         auto pick = [&](auto split_c) {
