@@ -12,16 +12,17 @@
 // connection -- coalesced along the row, XPF connections ahead -- and reused from a register
 // for all of the thread's slots; the accumulators acc[slot] stay in registers across the
 // alpha connections and are scatter-added into an LDS copy of the destination row once per
-// chunk.  Slot (io, iv) is orbital occ[io] annihilated / vir[iv] created, the order of the
-// single-excitation table, so the scatter reads (Ib, sign) straight from that table.  By
-// Hermiticity <Ib|E_tr|Jb> = <Jb|E_rt|Ib>, the table's sign for Jb, and (pq|rt) = (pq|tr).
+// chunk.  Slot (io, iv) is orbital occ[io] annihilated / vir[iv] created; its target rank
+// and sign are computed from the colex ranks (see the scatter), with <Ib|E_tr|Jb> the
+// parity of the occupied orbitals of Jb between r and t, and (pq|rt) = (pq|tr).
 //
 // The integral row W[r*n + t] = s_a (pq|rt) is stored R times interleaved (entry e, copy c at
 // e*R + c) and thread l reads copy l % R, so the lane-varying gathers W[occ*n + vir] of a
 // group of R lanes never share a bank.  W of the next connection is prefetched into
-// registers while the current one is consumed.  Contributions are added into the LDS row
-// one wave at a time, in wave order, so the sum order -- and the FP result -- is
-// deterministic.  Fixed block size, so fill positions are compile-time offsets.
+// registers while the current one is consumed.  With `ordered` the contributions are added
+// into the LDS row one wave at a time, in wave order, so the sum order -- and the FP result
+// -- is reproducible; otherwise all waves add at once.  Fixed block size, so fill positions
+// are compile-time offsets.
 // Included inside namespace hipotf after sx_* and u32/u64/i64 are defined.
 #pragma once
 

@@ -50,6 +50,9 @@ static void build_same_spin(SameSpinTable &T, const SpinSpace &S, const std::vec
     std::vector<int32_t> J((size_t)S.N * L);
     std::vector<double> C((size_t)S.N * L);
     std::vector<double> gd(S.N);
+    // P[I][g] = first entry of row I whose source index is >= range_start(g)
+    std::vector<int32_t> P((size_t)S.N * (SS_RANGES + 1));
+    auto range_start = [&](int g) { return (i64)g * S.N / SS_RANGES; };
     auto E = [&](int p, int q, int r, int s) { return eri[(((size_t)p * n + q) * n + r) * n + s]; };
     par_for(S.N, [&](i64 I) {
         const u64 s = S.strs[I];
@@ -101,10 +104,23 @@ static void build_same_spin(SameSpinTable &T, const SpinSpace &S, const std::vec
                         Jr[l] = (int32_t)host_find_str(S, js);
                         Cr[l] = (g1 ^ g2) ? -v : v;
                     }
+        // ascending source index, so a pass over one source range reads a contiguous piece
+        std::vector<std::pair<int32_t, double>> row(L);
+        for (size_t e = 0; e < L; ++e) row[e] = {Jr[e], Cr[e]};
+        std::sort(row.begin(), row.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+        int32_t *Pr = &P[(size_t)I * (SS_RANGES + 1)];
+        int rg = 0;
+        for (size_t e = 0; e < L; ++e) {
+            Jr[e] = row[e].first;
+            Cr[e] = row[e].second;
+            while (rg <= SS_RANGES && Jr[e] >= range_start(rg)) Pr[rg++] = (int32_t)e;
+        }
+        while (rg <= SS_RANGES) Pr[rg++] = (int32_t)L;
     });
     for (size_t i = 0; i < J.size(); ++i)
         HIPOTF_REQUIRE(J[i] >= 0 && J[i] < S.N, "internal: same-spin table index out of range");
     T.J.upload(J);
+    T.P.upload(P);
     T.c64.upload(C);
     std::vector<float> C32(C.begin(), C.end());
     T.c32.upload(C32);

@@ -140,9 +140,13 @@ DevBasis *make_dev_basis(const HostBasis &hb);
 // Hamiltonian in integral form (replaces the SVD network of src/include/ham/otf.hpp:114-169
 // for the 2-body H; the network constructor decodes a network into this form)
 // ------------------------------------------------------------------------------------------
+constexpr int SS_RANGES = 8;  // source-index ranges the same-spin rows are split into
+
 struct SameSpinTable {
-    int L = 0;                // 1 + nsx + ndx entries per dest string (diagonal first)
+    int L = 0;                // 1 + nsx + ndx entries per dest string, ascending source index
     DevBuf J;                 // int32[N][L]
+    DevBuf P;                 // int32[N][SS_RANGES + 1]: first entry of each source range
+                              // g (sources >= g * N / SS_RANGES); P[.][SS_RANGES] = L
     DevBuf c64, c32;          // double / float [N][L]
     DevBuf gdiag;             // double[N]  same-spin diagonal G[I,I]
 };

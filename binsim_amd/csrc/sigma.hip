@@ -63,16 +63,19 @@ __global__ __launch_bounds__(256) void k_gauge_in(SrcPack src, T *__restrict__ x
 }
 
 // ---------------------------------------------------------------------------------------
-// K2/K3: Y[I, :] = sum_l C[I,l] X[J[I,l], :]   (fixed-length CSR rows, uniform per wave)
+// K2/K3: Y[I, :] (+)= sum_l C[I,l] X[J[I,l], :]   (CSR rows sorted by source, uniform per wave)
 // One wave = one destination row x 64 consecutive (flattened) columns.  Blocks are mapped so
-// that each XCD walks all rows of one column tile before the next, keeping the tile's
-// source columns resident in that XCD's L2.
+// that each XCD walks all rows of one column tile before the next.  A launch covers only the
+// sources in ranges [g0, g1) (entries P[I][g0] .. P[I][g1]): the tile's source columns then
+// span 1/npass of the rows and stay resident in the XCD's 4 MB L2; npass launches sum up.
 // ---------------------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_rowspmm(const T *__restrict__ X, T *__restrict__ Y,
                                                  const int32_t *__restrict__ J,
+                                                 const int32_t *__restrict__ P,
                                                  const T *__restrict__ C, int L, i64 nrows,
-                                                 i64 ncols, i64 ntiles, i64 bpt) {
+                                                 i64 ncols, i64 ntiles, i64 bpt, int g0, int g1,
+                                                 int accumulate) {
     const int lane = threadIdx.x & 63;
     const i64 b = blockIdx.x;
     const i64 xcd = b & 7, j = b >> 3;
@@ -86,25 +89,26 @@ __global__ __launch_bounds__(256) void k_rowspmm(const T *__restrict__ X, T *__r
     const int32_t *Jr = J + row * (i64)L;
     const T *Cr = C + row * (i64)L;
     const T *Xc = X + col;
+    const int lb = P[row * (SS_RANGES + 1) + g0], le = P[row * (SS_RANGES + 1) + g1];
     T acc0 = 0, acc1 = 0;
-    int l = 0;
+    int l = lb;
     if (ok) {
         // U gathers in flight per wave; the next batch's (uniform) indices are loaded while
         // the current batch's gathers are outstanding
         constexpr int U = 16;
         int jn[U];
-        if (L >= U) {
+        if (l + U <= le) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) jn[u] = Jr[u];
+            for (int u = 0; u < U; ++u) jn[u] = Jr[l + u];
         }
-        for (; l + U <= L; l += U) {
+        for (; l + U <= le; l += U) {
             int jc[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) jc[u] = jn[u];
             T xv[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) xv[u] = Xc[(i64)jc[u] * ncols];
-            if (l + 2 * U <= L) {
+            if (l + 2 * U <= le) {
 #pragma unroll
                 for (int u = 0; u < U; ++u) jn[u] = Jr[l + U + u];
             }
@@ -114,8 +118,9 @@ __global__ __launch_bounds__(256) void k_rowspmm(const T *__restrict__ X, T *__r
                 acc1 += Cr[l + u + 1] * xv[u + 1];
             }
         }
-        for (; l < L; ++l) acc0 += Cr[l] * Xc[(i64)Jr[l] * ncols];
-        Y[row * ncols + col] = acc0 + acc1;
+        for (; l < le; ++l) acc0 += Cr[l] * Xc[(i64)Jr[l] * ncols];
+        T *y = Y + row * ncols + col;
+        *y = accumulate ? *y + (acc0 + acc1) : acc0 + acc1;
     }
 }
 
@@ -426,8 +431,19 @@ static void launch_rowspmm(const T *X, T *Y, const SameSpinTable &tab, const T *
     const i64 ntiles_pad = (ntiles + 7) / 8 * 8;
     const i64 nblocks = ntiles_pad * bpt;
     HIPOTF_REQUIRE(nblocks < (1ll << 31), "row-SpMM grid too large");
-    hipLaunchKernelGGL(k_rowspmm<T>, dim3((unsigned)nblocks), dim3(256), 0, st, X, Y,
-                       tab.J.as<int32_t>(), C, tab.L, nrows, ncols, ntiles, bpt);
+    // passes over source ranges (each pass's source columns of a tile span 1/npass of the
+    // rows).  Measured at 16 orbitals: 1, 2 and 4 passes take the same time -- the gathers
+    // are bound by the per-CU load pipeline (~31 B/clk), not by L2 misses -- so the default
+    // is one pass; HIPOTF_RS_PASSES = 2|4|8 (measurement)
+    static const int passes_env = [] {
+        const char *s = std::getenv("HIPOTF_RS_PASSES");
+        return s ? std::atoi(s) : 0;
+    }();
+    const int npass = passes_env == 2 || passes_env == 4 || passes_env == 8 ? passes_env : 1;
+    for (int p = 0; p < npass; ++p)
+        hipLaunchKernelGGL(k_rowspmm<T>, dim3((unsigned)nblocks), dim3(256), 0, st, X, Y,
+                           tab.J.as<int32_t>(), tab.P.as<int32_t>(), C, tab.L, nrows, ncols, ntiles,
+                           bpt, p * SS_RANGES / npass, (p + 1) * SS_RANGES / npass, p > 0 ? 1 : 0);
     HIPOTF_CHECK(hipGetLastError());
 }
 
@@ -588,7 +604,12 @@ static void launch_ab(DevHam *H, const T *xt, T *s1, int nv, const T *eri, hipSt
     int v0 = 0;
     if (ab_kernel_pref() <= 1) {
         while (v0 < nv) {
-            if (nv - v0 >= 2 && try_ab_src<T, 2>(H, xt, s1, nv, v0, eri, st)) { v0 += 2; continue; }
+            // HIPOTF_AB_NVS=1: one vector per launch (measurement)
+            static const bool one = [] {
+                const char *s = std::getenv("HIPOTF_AB_NVS");
+                return s && std::atoi(s) == 1;
+            }();
+            if (!one && nv - v0 >= 2 && try_ab_src<T, 2>(H, xt, s1, nv, v0, eri, st)) { v0 += 2; continue; }
             if (try_ab_src<T, 1>(H, xt, s1, nv, v0, eri, st)) { v0 += 1; continue; }
             break;
         }
