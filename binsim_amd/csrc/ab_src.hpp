@@ -29,8 +29,10 @@
 constexpr int AB_SRC_NJ = 512;   // source strings per chunk
 constexpr int AB_SRC_NWE = 256;  // integral entries one W fill covers (n*n <= 256)
 constexpr int AB_SRC_XPF = 3;    // alpha connections of source elements in flight
+constexpr int AB_SRC_BINOM = 160; // binomial table entries: n <= 16 orbitals x (KM + 2) <= 10
+constexpr int AB_SRC_CONN = 64;   // alpha connections per row: k (n - k) <= 64 for n <= 16
 
-template <typename T, int NVS, int KM, int VM, int R, int SPLIT>
+template <typename T, int NVS, int KM, int VM, int R, int SPLIT, int DB>
 __global__ __launch_bounds__(AB_SRC_NJ *SPLIT) __attribute__((amdgpu_waves_per_eu(2 * SPLIT, 2 * SPLIT)))
 void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, const u32 *__restrict__ sxA,
               const u32 *__restrict__ sxB, const u64 *__restrict__ strsA, const u64 *__restrict__ strsB,
@@ -42,11 +44,17 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
     static_assert(KM % SPLIT == 0 && (AB_SRC_NWE * R) % BD == 0 && BD % R == 0, "fill layout");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n2 = n * n;
-    T *Wr = reinterpret_cast<T *>(smem_raw);       // AB_SRC_NWE * R replicated integral row
-    T *Wdg = Wr + AB_SRC_NWE * R;                  // AB_SRC_NWE + 16: sum_{p in Ia} (pp|rt), zero padded
+    // DB = 2: two replicated rows, the next connection's filled while the current one is read
+    // (one barrier per connection instead of two)
+    T *Wr = reinterpret_cast<T *>(smem_raw);       // DB x AB_SRC_NWE * R replicated integral rows
+    T *Wdg = Wr + DB * AB_SRC_NWE * R;             // AB_SRC_NWE + 16: sum_{p in Ia} (pp|rt), zero padded
     T *S = Wdg + AB_SRC_NWE + 16;                  // NB * NVS destination row
     constexpr int BW = KM + 2;
     int *binom = reinterpret_cast<int *>(S + NB * NVS);  // C(p, j), p < n, j < BW
+    // the row's alpha connections (single-excitation table entries of Ia), staged so the round
+    // loop issues no scalar memory loads: those share the LDS wait counter and would force
+    // every wait on the slot gathers to drain all of them
+    u32 *conn = reinterpret_cast<u32 *>(binom + AB_SRC_BINOM);
     const int tid = threadIdx.x;
     const int jl = tid & (AB_SRC_NJ - 1);          // source string within the chunk
     const int half = SPLIT == 1 ? 0 : __builtin_amdgcn_readfirstlane(tid / AB_SRC_NJ);
@@ -71,6 +79,7 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
         for (int i = 0; i < j && c; ++i) c = c * (u64)(p - i) / (u64)(i + 1);
         binom[t] = (int)c;
     }
+    for (int t = tid; t < nsxA; t += BD) conn[t] = sxA[(i64)t * NA + Ia];
     __syncthreads();
 
     const int nca = nsxA + 1;
@@ -84,7 +93,7 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
     // zero padding after the table and land in Wr slots no lane reads.
     auto load_w = [&](T (&pre)[NFM], bool &neg, int ca) {
         if (ca < nsxA) {
-            const u32 e = sxA[(i64)ca * NA + Ia];
+            const u32 e = conn[ca];
             const T *er = eri + (i64)(sx_r(e) * n + sx_s(e)) * n2 + e0;
             neg = sx_neg(e);
 #pragma unroll
@@ -100,7 +109,7 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
     auto load_x = [&](T (&xv)[NVS], i64 chunk, int ca) {
         if (chunk >= nchunks) return;
         const i64 Jb = chunk * AB_SRC_NJ + jl;
-        const i64 Ja = ca < nsxA ? (i64)sx_J(sxA[(i64)ca * NA + Ia]) : Ia;
+        const i64 Ja = ca < nsxA ? (i64)sx_J(conn[ca]) : Ia;
 #pragma unroll
         for (int v = 0; v < NVS; ++v) xv[v] = Jb < NB ? xt[(Ja * NB + Jb) * nvst + v0 + v] : T(0);
     };
@@ -108,6 +117,16 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
     T pre[NFM];
     bool pneg = false;
     load_w(pre, pneg, 0);
+    int wca = 0;  // connection whose W values are in pre
+    if (DB == 2) {
+        // round 0's row goes into buffer 0 now; pre then carries round 1's
+#pragma unroll
+        for (int f = 0; f < NFM; ++f) Wr[tid + f * BD] = pneg ? -pre[f] : pre[f];
+        if (nit > 1) {
+            wca = 1 % nca;
+            load_w(pre, pneg, wca);
+        }
+    }
     T xq[XPF][NVS];
     i64 pchunk = 0;  // round whose source elements are loaded next
     int pca = 0;
@@ -130,7 +149,7 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
     bool live = false;
     // slots beyond (kB, vB) of the instantiated KM x VM grid read valid LDS words into
     // accumulators that are never scattered; their diagonal terms read the zero padding
-    const u32 zero_off = (u32)((AB_SRC_NWE * R + AB_SRC_NWE) * sizeof(T));
+    const u32 zero_off = (u32)((DB * AB_SRC_NWE * R + AB_SRC_NWE) * sizeof(T));
     i64 chunk = 0;
     int ca = 0;
     for (i64 it = 0; it < nit; ++it, advance(chunk, ca)) {
@@ -162,41 +181,65 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
 #pragma unroll
             for (int v = 0; v < NVS; ++v) accd[v] = T(0);
         }
-        __syncthreads();  // everyone is done reading the previous W
+        u32 wr_off = 0;  // byte offset of this round's replicated row
+        if (DB == 2) {
+            // this round's row (filled last round) is complete and the other buffer's last
+            // readers are done: fill it with the next round's row, load the one after
+            __syncthreads();
+            wr_off = (u32)((it & 1) * AB_SRC_NWE * R * sizeof(T));
+            if (it + 1 < nit) {
+                T *Wn = Wr + ((it + 1) & 1) * AB_SRC_NWE * R;
 #pragma unroll
-        for (int f = 0; f < NFM; ++f) Wr[tid + f * BD] = pneg ? -pre[f] : pre[f];
-        __syncthreads();
-        if (it + 1 < nit) load_w(pre, pneg, ca + 1 == nca ? 0 : ca + 1);
+                for (int f = 0; f < NFM; ++f) Wn[tid + f * BD] = pneg ? -pre[f] : pre[f];
+                if (it + 2 < nit) {
+                    wca = wca + 1 == nca ? 0 : wca + 1;
+                    load_w(pre, pneg, wca);
+                }
+            }
+        } else {
+            __syncthreads();  // everyone is done reading the previous W
+#pragma unroll
+            for (int f = 0; f < NFM; ++f) Wr[tid + f * BD] = pneg ? -pre[f] : pre[f];
+            __syncthreads();
+            if (it + 1 < nit) load_w(pre, pneg, ca + 1 == nca ? 0 : ca + 1);
+        }
         T xnew[NVS];
 #pragma unroll
         for (int v = 0; v < NVS; ++v) xnew[v] = T(0);
         load_x(xnew, pchunk, pca);
         advance(pchunk, pca);
         {
-            // the empty asm makes the row offsets opaque per iteration, so the KH*VM slot
-            // addresses are formed in the loop (one add each) instead of hoisted into VGPRs
+            // a zero read from LDS after this round's barrier makes the row offsets opaque per
+            // round, so the KH*VM slot addresses are formed in the loop (one add each) instead
+            // of being hoisted into VGPRs (inline asm would do it too, but forces full waits)
+            const u32 lz = *reinterpret_cast<const u32 *>(smem_raw + zero_off);
             u32 ob[KH];
 #pragma unroll
-            for (int i = 0; i < KH; ++i) {
-                ob[i] = oo[i];
-                asm("" : "+v"(ob[i]) : "s"(ca));
-            }
-            // one row of VM gathers in flight before its FMAs; the scheduling barrier keeps
-            // the compiler from overlapping rows (more gathers in flight would push the
-            // accumulators out of registers)
+            for (int i = 0; i < KH; ++i) ob[i] = oo[i] + wr_off + lz;
+            // software pipeline over the slot rows: row i+1's VM gathers are issued before row
+            // i's FMAs, so the LDS latency of one row hides behind the FMAs of the previous
+            // one; the scheduling barrier keeps the compiler from hoisting more rows of
+            // gathers (their registers would push the accumulators out)
+            T wv[2][VM];
+#pragma unroll
+            for (int iv = 0; iv < VM; ++iv) wv[0][iv] = *reinterpret_cast<const T *>(smem_raw + ob[0] + vv[iv]);
 #pragma unroll
             for (int i = 0; i < KH; ++i) {
-                T wv[VM];
+                if (i + 1 < KH) {
 #pragma unroll
-                for (int iv = 0; iv < VM; ++iv) wv[iv] = *reinterpret_cast<const T *>(smem_raw + ob[i] + vv[iv]);
+                    for (int iv = 0; iv < VM; ++iv)
+                        wv[(i + 1) & 1][iv] = *reinterpret_cast<const T *>(smem_raw + ob[i + 1] + vv[iv]);
+                }
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int iv = 0; iv < VM; ++iv)
 #pragma unroll
-                    for (int v = 0; v < NVS; ++v) acc[i][iv][v] += wv[iv] * xq[0][v];
+                    for (int v = 0; v < NVS; ++v) acc[i][iv][v] += wv[i & 1][iv] * xq[0][v];
             }
             T wd = 0;
 #pragma unroll
-            for (int i = 0; i < KH; ++i) wd += *reinterpret_cast<const T *>(smem_raw + dd[i]);
+            for (int i = 0; i < KH; ++i)
+                wd += *reinterpret_cast<const T *>(smem_raw + dd[i] + (half * KH + i < kB ? wr_off : 0u));
 #pragma unroll
             for (int v = 0; v < NVS; ++v) accd[v] += wd * xq[0][v];
         }

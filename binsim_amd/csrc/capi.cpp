@@ -32,7 +32,7 @@ struct DavidsonOut {
     int refine_iter = -1;
     int converged = 0;
 };
-DavidsonOut davidson(DevHam *H, int nroots, int mode, double tol, double switch_tol, int maxiter,
+DavidsonOut davidson(DevHam *H, int nroots, int mode, int block, double tol, double switch_tol, int maxiter,
                      int maxspace, int verbose, const double *guess, double *energies, double *vecs);
 
 // ------------------------------------------------------------------------------------------
@@ -540,15 +540,15 @@ int hipotf_vqe_energy_grad(void *basis, void *ham, void *pool, int64_t nparams, 
 // ---- Davidson driver (jl/davidson.jl outer loop on the device; see davidson.cpp) -----------
 // info (host, 7 doubles): iterations, fp64 sigma vectors, fp32 sigma vectors, max residual,
 // wall seconds, iteration of the fp32 -> fp64 switch (-1 if none), converged flag
-int hipotf_davidson_f64(void *basis, void *ham, int nroots, int mode, double tol, double switch_tol,
-                        int maxiter, int maxspace, int verbose, const double *guess, double *energies,
-                        double *vecs, double *info) {
+int hipotf_davidson_f64(void *basis, void *ham, int nroots, int mode, int block, double tol,
+                        double switch_tol, int maxiter, int maxspace, int verbose, const double *guess,
+                        double *energies, double *vecs, double *info) {
     API_TRY
     HIPOTF_REQUIRE(energies && info, "NULL output");
     HIPOTF_REQUIRE(mode == 0 || mode == 1, "mode must be 0 (fp64) or 1 (mixed block)");
     DevBasis *B = as_basis(basis);
     DevHam *H = as_net(ham)->ham_for(B);
-    DavidsonOut o = davidson(H, nroots, mode, tol, switch_tol, maxiter, maxspace, verbose, guess,
+    DavidsonOut o = davidson(H, nroots, mode, block, tol, switch_tol, maxiter, maxspace, verbose, guess,
                              energies, vecs);
     info[0] = o.iterations;
     info[1] = (double)o.sigma64;

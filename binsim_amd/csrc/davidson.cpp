@@ -5,11 +5,13 @@
 //   diagonal preconditioner with shift max(|r| 1e-2, 1e-4), modified Gram-Schmidt repeated
 //   when the norm drops below 0.717, normalisation -- extended to nroots roots (one correction
 //   vector per unconverged root per iteration, Davidson-Liu).  Every sigma is one FP64 vector.
-// mode 1 ("mixed block"): the same iteration with all correction vectors of an iteration
-//   applied in one block sigma launch in FP32 (coefficients, integrals and work vectors), the
-//   subspace, Rayleigh-Ritz, residuals and energies in FP64; when every residual is below
-//   switch_tol (or the FP32 residuals stop improving) the Ritz vectors seed an FP64 phase
-//   (FP64 block sigma) that runs to tol.
+// mode 1 ("mixed block"): block Davidson -- the k = max(block, nroots) lowest Ritz pairs are
+//   corrected every iteration (the extra pairs accelerate the wanted ones) and all correction
+//   vectors of an iteration go through one block sigma launch in FP32 (coefficients,
+//   integrals and work vectors); the subspace, Rayleigh-Ritz, residuals and energies stay
+//   FP64; restarts keep half the subspace (thick restart).  When the wanted residuals are
+//   below switch_tol (or the FP32 residuals stop improving) the k Ritz vectors seed an FP64
+//   phase (FP64 block sigma) that runs to tol.
 // The small symmetric eigenproblem (m <= 64) is solved on the host by cyclic Jacobi.
 #include "hipotf.hpp"
 
@@ -88,25 +90,31 @@ struct DavidsonOut {
     int converged = 0;
 };
 
-DavidsonOut davidson(DevHam *H, int nroots, int mode, double tol, double switch_tol, int maxiter,
+DavidsonOut davidson(DevHam *H, int nroots, int mode, int block, double tol, double switch_tol, int maxiter,
                      int maxspace, int verbose, const double *guess, double *energies, double *vecs) {
     const auto t0 = std::chrono::steady_clock::now();
     hipStream_t st = (hipStream_t)0;
     const i64 N = H->basis->host.dim;
     HIPOTF_REQUIRE(N > 0, "empty basis");
     HIPOTF_REQUIRE(nroots >= 1 && nroots <= 16 && nroots <= N, "nroots out of range");
-    if (maxspace <= 0) maxspace = nroots == 1 ? 23 : 8 * nroots;
+    // k = Ritz pairs corrected per iteration: nroots in FP64 mode (jl/davidson.jl), at least 4
+    // in the mixed block mode (the extra pairs speed up the wanted ones)
+    const int k = mode == 0 ? nroots : (int)std::min<i64>(std::max(block > 0 ? block : 4, nroots), 16);
+    if (maxspace <= 0) maxspace = mode == 0 ? (nroots == 1 ? 23 : 8 * nroots) : 16 * k;
     maxspace = (int)std::min<i64>(std::min(maxspace, 64), N);
-    // vectors kept on restart: ncv = 3 for one root (jl/davidson.jl), 4 per root otherwise
-    const int nkeep = nroots == 1 ? std::min(3, maxspace - 1) : std::max(nroots, std::min(4 * nroots, maxspace / 2));
-    HIPOTF_REQUIRE(maxspace >= 2 * nroots || maxspace == N, "maxspace must be >= 2 * nroots");
+    // vectors kept on restart: ncv = 3 for one root (jl/davidson.jl), 4 per root otherwise;
+    // the block mode keeps half the subspace (thick restart)
+    const int nkeep = mode == 1 ? std::min(maxspace - k, std::max(2 * k, maxspace / 2))
+                      : nroots == 1 ? std::min(3, maxspace - 1)
+                                    : std::max(nroots, std::min(4 * nroots, maxspace / 2));
+    HIPOTF_REQUIRE(maxspace >= 2 * k || maxspace == N, "maxspace must be >= 2 * block");
     DavidsonOut out;
 
-    std::vector<DevBuf> Vb(maxspace), AVb(maxspace), R(nroots);
+    std::vector<DevBuf> Vb(maxspace), AVb(maxspace), R(k);
     for (auto &b : Vb) b.alloc(N * sizeof(double));
     for (auto &b : AVb) b.alloc(N * sizeof(double));
     for (auto &b : R) b.alloc(N * sizeof(double));
-    DevBuf diag(N * sizeof(double)), tmp(N * sizeof(double) * (size_t)nroots);
+    DevBuf diag(N * sizeof(double)), tmp(N * sizeof(double) * (size_t)k);
     HIPOTF_CHECK(hipMemsetAsync(diag.p, 0, N * sizeof(double), st));
     diag_elements(H, diag.as<double>(), st);
     std::vector<double *> V(maxspace), AV(maxspace);
@@ -115,33 +123,49 @@ DavidsonOut davidson(DevHam *H, int nroots, int mode, double tol, double switch_
         AV[i] = AVb[i].as<double>();
     }
 
-    // initial guesses: caller's vectors, else unit vectors on the nroots lowest diagonals
-    if (guess) {
-        for (int i = 0; i < nroots; ++i)
-            HIPOTF_CHECK(hipMemcpyAsync(V[i], guess + (size_t)i * N, N * sizeof(double),
-                                        hipMemcpyDeviceToDevice, st));
-    } else {
+    // initial subspace: the caller's nroots vectors, else unit vectors on the lowest diagonals;
+    // the block mode tops it up to k vectors with further lowest-diagonal unit vectors
+    std::vector<i64> low;
+    {
         std::vector<double> dh(N);
         HIPOTF_CHECK(hipMemcpy(dh.data(), diag.p, N * sizeof(double), hipMemcpyDeviceToHost));
         std::vector<i64> idx(N);
         std::iota(idx.begin(), idx.end(), 0);
-        std::partial_sort(idx.begin(), idx.begin() + nroots, idx.end(),
+        const i64 nl = std::min<i64>(N, (i64)k + nroots);
+        std::partial_sort(idx.begin(), idx.begin() + nl, idx.end(),
                           [&](i64 a, i64 b) { return dh[a] < dh[b] || (dh[a] == dh[b] && a < b); });
-        const double one = 1.0;
-        for (int i = 0; i < nroots; ++i) {
-            HIPOTF_CHECK(hipMemsetAsync(V[i], 0, N * sizeof(double), st));
-            HIPOTF_CHECK(hipMemcpyAsync(V[i] + idx[i], &one, sizeof(double), hipMemcpyHostToDevice, st));
-            HIPOTF_CHECK(hipStreamSynchronize(st));
-        }
+        low.assign(idx.begin(), idx.begin() + nl);
     }
+    const double one = 1.0;
     int m = 0;
-    for (int i = 0; i < nroots; ++i) {
-        double nb = std::sqrt(dot(N, V[i], V[i], st));
-        double na = mgs(N, m, (const double *const *)V.data(), V[i], st);
-        if (na < 0.717 * nb) na = mgs(N, m, (const double *const *)V.data(), V[i], st);
-        HIPOTF_REQUIRE(na > 1e-12, "initial guesses are linearly dependent");
-        normalize(N, V[i], st);
+    auto add_unit = [&](i64 at) {
+        HIPOTF_CHECK(hipMemsetAsync(V[m], 0, N * sizeof(double), st));
+        HIPOTF_CHECK(hipMemcpyAsync(V[m] + at, &one, sizeof(double), hipMemcpyHostToDevice, st));
+        HIPOTF_CHECK(hipStreamSynchronize(st));
+    };
+    auto accept = [&](bool required) {  // orthonormalise V[m] against V[0..m); keep if independent
+        const double nb = std::sqrt(dot(N, V[m], V[m], st));
+        double na = mgs(N, m, (const double *const *)V.data(), V[m], st);
+        if (na < 0.717 * nb) na = mgs(N, m, (const double *const *)V.data(), V[m], st);
+        if (na <= 1e-12 * std::max(1.0, nb)) {
+            HIPOTF_REQUIRE(!required, "initial guesses are linearly dependent");
+            return;
+        }
+        normalize(N, V[m], st);
         ++m;
+    };
+    size_t next_low = 0;
+    for (int i = 0; i < nroots; ++i) {
+        if (guess)
+            HIPOTF_CHECK(hipMemcpyAsync(V[m], guess + (size_t)i * N, N * sizeof(double),
+                                        hipMemcpyDeviceToDevice, st));
+        else
+            add_unit(low[next_low++]);
+        accept(true);
+    }
+    while (m < (mode == 1 ? k : nroots) && next_low < low.size()) {
+        add_unit(low[next_low++]);
+        accept(false);
     }
     int phase = mode == 1 ? 1 : 0;  // 1: fp32 block sigma, 0: fp64
     auto apply = [&](int first, int count) {
@@ -171,7 +195,7 @@ DavidsonOut davidson(DevHam *H, int nroots, int mode, double tol, double switch_
     set_cols(0, m);
 
     std::vector<double> w, C, best(3, 1e300);
-    std::vector<double> rn(nroots);
+    std::vector<double> rn(k);
     int stall = 0;
     for (int it = 1; it <= maxiter; ++it) {
         out.iterations = it;
@@ -179,14 +203,16 @@ DavidsonOut davidson(DevHam *H, int nroots, int mode, double tol, double switch_
         for (int r = 0; r < m; ++r)
             for (int c = 0; c < m; ++c) a[(size_t)r * m + c] = heff[(size_t)r * maxspace + c];
         jacobi_eigh(m, a, w, C);
-        std::vector<std::vector<double>> coef(nroots, std::vector<double>(m));
-        double rmax = 0.0;
-        for (int i = 0; i < nroots; ++i) {
+        const int kk = std::min(k, m);  // Ritz pairs with residuals this iteration
+        std::vector<std::vector<double>> coef(kk, std::vector<double>(m));
+        double rmax = 0.0;  // over the wanted roots
+        for (int i = 0; i < kk; ++i) {
             for (int j = 0; j < m; ++j) coef[i][j] = C[(size_t)j * m + i];
             rn[i] = ritz_and_residual(N, m, coef[i].data(), w[i], (const double *const *)V.data(),
                                       (const double *const *)AV.data(), R[i].as<double>(), st);
-            rmax = std::max(rmax, rn[i]);
+            if (i < nroots) rmax = std::max(rmax, rn[i]);
         }
+        for (int i = 0; i < nroots; ++i) energies[i] = w[i];  // current estimates
         out.max_residual = rmax;
         if (verbose) {  // progress goes to stderr: stdout belongs to the caller
             const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -207,15 +233,15 @@ DavidsonOut davidson(DevHam *H, int nroots, int mode, double tol, double switch_
         }
         if (rmax < cur_tol) {
             if (phase == 1) {
-                // Ritz vectors -> new subspace, FP64 sigma from here on
-                for (int i = 0; i < nroots; ++i)
+                // the kk lowest Ritz vectors -> new subspace, FP64 sigma from here on
+                for (int i = 0; i < kk; ++i)
                     ritz_vector(N, m, coef[i].data(), (const double *const *)V.data(),
                                 tmp.as<double>() + (size_t)i * N, st);
-                for (int i = 0; i < nroots; ++i)
+                for (int i = 0; i < kk; ++i)
                     HIPOTF_CHECK(hipMemcpyAsync(V[i], tmp.as<double>() + (size_t)i * N, N * sizeof(double),
                                                 hipMemcpyDeviceToDevice, st));
                 m = 0;
-                for (int i = 0; i < nroots; ++i) {
+                for (int i = 0; i < kk; ++i) {
                     mgs(N, m, (const double *const *)V.data(), V[i], st);
                     normalize(N, V[i], st);
                     ++m;
@@ -233,9 +259,10 @@ DavidsonOut davidson(DevHam *H, int nroots, int mode, double tol, double switch_
             out.converged = 1;
             break;
         }
-        // correction vectors for the unconverged roots
+        // correction vectors for the unconverged Ritz pairs (the wanted roots, plus the extra
+        // pairs of the block mode)
         std::vector<int> todo;
-        for (int i = 0; i < nroots; ++i)
+        for (int i = 0; i < kk; ++i)
             if (rn[i] >= cur_tol) todo.push_back(i);
         if (m + (int)todo.size() > maxspace) {
             // restart to the nkeep lowest Ritz vectors (jl/davidson.jl:82-114)
@@ -274,7 +301,6 @@ DavidsonOut davidson(DevHam *H, int nroots, int mode, double tol, double switch_
         }
         if (m == first) {
             // breakdown: no new direction (jl/davidson.jl:135-143)
-            for (int i = 0; i < nroots; ++i) energies[i] = w[i];
             if (vecs)
                 for (int i = 0; i < nroots; ++i)
                     ritz_vector(N, std::min(m, (int)coef[i].size()), coef[i].data(),

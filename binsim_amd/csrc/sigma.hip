@@ -513,12 +513,23 @@ static bool try_ab_src(DevHam *H, const T *xt, T *s1, int nvst, int v0, const T 
         const int split = split_env == 2 ? 2 : 1;
         const i64 NB = Bs.N;
         const int bd = AB_SRC_NJ * split;
-        if (n2 > AB_SRC_NWE) return false;
+        if (n2 > AB_SRC_NWE || A.nsx > AB_SRC_CONN || n * (8 + 2) > AB_SRC_BINOM) return false;
         // a W fill reads AB_SRC_NWE entries from the start of one (pq|..) row
         static_assert(AB_SRC_NWE <= ERI_PAD + 1, "integral tables are padded for one fill past the last row");
-        // + the binomial table C(p, j), p < n <= 16, j < KM + 2 <= 10
-        const size_t smem = ((size_t)AB_SRC_NWE * R + AB_SRC_NWE + 16 + (size_t)NB * NVS) * sizeof(T) + 16 * 10 * 4;
-        if (smem > std::min<size_t>(lds_budget(), 160 * 1024)) return false;
+        // + the binomial table C(p, j), p < n <= 16, j < KM + 2 <= 10; two replicated rows
+        // (one barrier per connection) when they fit, HIPOTF_AB_DB = 1 forces one (measurement)
+        auto lds_for = [&](int db) {
+            return ((size_t)db * AB_SRC_NWE * R + AB_SRC_NWE + 16 + (size_t)NB * NVS) * sizeof(T) +
+                   (AB_SRC_BINOM + AB_SRC_CONN) * 4;
+        };
+        static const int db_env = [] {
+            const char *s = std::getenv("HIPOTF_AB_DB");
+            return s ? std::atoi(s) : 0;
+        }();
+        const size_t lds_max = std::min<size_t>(lds_budget(), 160 * 1024);
+        const int db = (db_env != 1 && lds_for(2) <= lds_max) ? 2 : 1;
+        const size_t smem = lds_for(db);
+        if (smem > lds_max) return false;
         const i64 per = (A.N + 7) / 8;
         const i64 nblocks = per * 8;
         HIPOTF_REQUIRE(nblocks < (1ll << 31), "alpha-beta grid too large");
@@ -530,16 +541,19 @@ static bool try_ab_src(DevHam *H, const T *xt, T *s1, int nvst, int v0, const T 
                                n, A.N, NB, A.nsx, kB, vB, deterministic() ? 1 : 0);
             HIPOTF_CHECK(hipGetLastError());
         };
-        auto pick = [&](auto split_c) {
-            constexpr int SPLIT = decltype(split_c)::value;
-            if (kB <= 4 && vB <= 4) go(k_ab_src<T, NVS, 4, 4, R, SPLIT>);
-            else if (kB <= 6 && vB <= 6) go(k_ab_src<T, NVS, 6, 6, R, SPLIT>);
+        auto pick = [&](auto split_c, auto db_c) {
+            constexpr int SPLIT = decltype(split_c)::value, DB = decltype(db_c)::value;
+            if (kB <= 4 && vB <= 4) go(k_ab_src<T, NVS, 4, 4, R, SPLIT, DB>);
+            else if (kB <= 6 && vB <= 6) go(k_ab_src<T, NVS, 6, 6, R, SPLIT, DB>);
             else if constexpr (sizeof(T) == 8 && NVS == 2) return false;  // 128 FP64 accumulators: spills
-            else if (kB <= 8 && vB <= 8) go(k_ab_src<T, NVS, 8, 8, R, SPLIT>);
+            else if (kB <= 8 && vB <= 8) go(k_ab_src<T, NVS, 8, 8, R, SPLIT, DB>);
             else return false;
             return true;
         };
-        return split == 2 ? pick(std::integral_constant<int, 2>{}) : pick(std::integral_constant<int, 1>{});
+        using I1 = std::integral_constant<int, 1>;
+        using I2 = std::integral_constant<int, 2>;
+        if (split == 2) return db == 2 ? pick(I2{}, I2{}) : pick(I2{}, I1{});
+        return db == 2 ? pick(I1{}, I2{}) : pick(I1{}, I1{});
     }
 }
 

@@ -95,7 +95,7 @@ def davidson(aop, v0, diags, tol: float = 1e-5, ncv: int = 3, maxspace: int | No
 
 def solve(basis, ham, nroots: int = 1, mixed: bool = False, tol: float = 1e-8,
           switch_tol: float = 1e-5, maxiter: int = 500, maxspace: int = 0, verbose: bool = False,
-          guess=None, want_vectors: bool = False):
+          guess=None, want_vectors: bool = False, block: int = 0):
     """Device-resident Davidson (hipotf_davidson_f64).  Returns (energies, vectors, info)."""
     import torch
 
@@ -104,7 +104,7 @@ def solve(basis, ham, nroots: int = 1, mixed: bool = False, tol: float = 1e-8,
     e = np.zeros(nroots)
     info = np.zeros(7)
     vecs = torch.empty(nroots * dim, dtype=torch.float64, device="cuda") if want_vectors else None
-    rc = lib.hipotf_davidson_f64(basis.ptr, ham.ptr, nroots, 1 if mixed else 0, tol, switch_tol,
+    rc = lib.hipotf_davidson_f64(basis.ptr, ham.ptr, nroots, 1 if mixed else 0, block, tol, switch_tol,
                                  maxiter, maxspace, 1 if verbose else 0,
                                  0 if guess is None else int(guess.data_ptr()), e.ctypes.data_as(L._dp),
                                  0 if vecs is None else int(vecs.data_ptr()), info.ctypes.data_as(L._dp))

@@ -110,16 +110,18 @@ int hipotf_vqe_energy_grad(void *basis, void *ham, void *pool, int64_t nparams, 
 
 /* Davidson solve on the device (outer loop of jl/davidson.jl:1-185 extended to nroots roots;
  * no reference counterpart runs on the GPU).  mode 0: FP64, one FP64 sigma per correction
- * vector (the drop-in arithmetic); mode 1: mixed-precision block -- all correction vectors of
- * an iteration in one FP32 block sigma, FP64 subspace / Rayleigh-Ritz / residuals, then FP64
- * block refinement once every residual < switch_tol (or the FP32 residuals stagnate) until
- * tol.  guess: device [nroots*dim] or NULL (unit vectors on the lowest diagonals);
- * energies: host [nroots]; vecs: device [nroots*dim] or NULL; info: host [7] = {iterations,
- * fp64 sigma vectors, fp32 sigma vectors, max residual, seconds, switch iteration, converged}.
- * maxspace <= 0 selects 23 for one root (ncv + 20) and 8*nroots otherwise.  Returns 0 when
- * converged, 2 when maxiter was reached, 1 on error. */
-int hipotf_davidson_f64(void *basis, void *ham, int nroots, int mode, double tol, double switch_tol,
-                        int maxiter, int maxspace, int verbose, const double *guess,
+ * vector (the drop-in arithmetic, restart as jl/davidson.jl); mode 1: mixed-precision block
+ * -- the `block` (<= 0: max(4, nroots)) lowest Ritz pairs are corrected each iteration in one
+ * FP32 block sigma, FP64 subspace / Rayleigh-Ritz / residuals, thick restart to half the
+ * subspace, then FP64 block refinement once the wanted residuals < switch_tol (or the FP32
+ * residuals stagnate) until tol.  guess: device [nroots*dim] or NULL (unit vectors on the
+ * lowest diagonals); energies: host [nroots] (the latest estimates also when not
+ * converged); vecs: device [nroots*dim] or NULL; info: host [7] = {iterations, fp64 sigma
+ * vectors, fp32 sigma vectors, max residual, seconds, switch iteration, converged}.
+ * maxspace <= 0 selects 23 for one root (ncv + 20) and 8*nroots in mode 0, 16*block in
+ * mode 1 (at most 64).  Returns 0 when converged, 2 when maxiter was reached, 1 on error. */
+int hipotf_davidson_f64(void *basis, void *ham, int nroots, int mode, int block, double tol,
+                        double switch_tol, int maxiter, int maxspace, int verbose, const double *guess,
                         double *energies, double *vecs, double *info);
 
 /* ---- Davidson inner loop on the device: replaces libdiag.so (src/include/diag/declare.hpp).
