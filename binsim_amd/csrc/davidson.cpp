@@ -30,6 +30,12 @@ double mgs(i64 N, int dim, const double *const *V, double *vt, hipStream_t st);
 double normalize(i64 N, double *vec, hipStream_t st);
 double dot(i64 N, const double *a, const double *b, hipStream_t st);
 void multidot(i64 N, int dim, const double *const *V, const double *a, double *out_host, hipStream_t st);
+void block_combine(i64 N, int m, int k, const double *C, const double *w, const double *const *X,
+                   const double *const *Y, double *const *out, double *norms, hipStream_t st);
+void block_dots(i64 N, int na, const double *const *A, int nbv, const double *const *B, double *G,
+                hipStream_t st);
+void block_subtract(i64 N, int m, const double *const *V, const double *P, int nt, double *const *T,
+                    hipStream_t st);
 
 // eigen-decomposition of a symmetric m x m matrix (row-major a), ascending eigenvalues,
 // eigenvectors in the columns of v (row-major)
@@ -186,6 +192,20 @@ DavidsonOut davidson(DevHam *H, int nroots, int mode, int block, double tol, dou
     apply(0, m);
     std::vector<double> heff((size_t)maxspace * maxspace, 0.0);
     auto set_cols = [&](int first, int count) {
+        if (mode == 1) {
+            // columns first..first+count of V^T AV, up to 8 at a time, one pass over V each
+            for (int c0 = first; c0 < first + count; c0 += 8) {
+                const int nc = std::min(8, first + count - c0), nr = c0 + nc;
+                std::vector<double> G((size_t)nr * nc);
+                block_dots(N, nr, (const double *const *)V.data(), nc, (const double *const *)&AV[c0], G.data(), st);
+                for (int j = 0; j < nc; ++j) {
+                    const int c = c0 + j;
+                    for (int r = 0; r <= c; ++r)
+                        heff[(size_t)r * maxspace + c] = heff[(size_t)c * maxspace + r] = G[(size_t)r * nc + j];
+                }
+            }
+            return;
+        }
         std::vector<double> col(maxspace);
         for (int c = first; c < first + count; ++c) {
             multidot(N, c + 1, (const double *const *)V.data(), AV[c], col.data(), st);
@@ -206,12 +226,23 @@ DavidsonOut davidson(DevHam *H, int nroots, int mode, int block, double tol, dou
         const int kk = std::min(k, m);  // Ritz pairs with residuals this iteration
         std::vector<std::vector<double>> coef(kk, std::vector<double>(m));
         double rmax = 0.0;  // over the wanted roots
-        for (int i = 0; i < kk; ++i) {
+        for (int i = 0; i < kk; ++i)
             for (int j = 0; j < m; ++j) coef[i][j] = C[(size_t)j * m + i];
-            rn[i] = ritz_and_residual(N, m, coef[i].data(), w[i], (const double *const *)V.data(),
-                                      (const double *const *)AV.data(), R[i].as<double>(), st);
-            if (i < nroots) rmax = std::max(rmax, rn[i]);
+        if (mode == 1) {
+            // all kk residuals in one pass over the subspace
+            std::vector<double> Cm((size_t)m * kk);
+            std::vector<double *> Rp(kk);
+            for (int j = 0; j < m; ++j)
+                for (int i = 0; i < kk; ++i) Cm[(size_t)j * kk + i] = coef[i][j];
+            for (int i = 0; i < kk; ++i) Rp[i] = R[i].as<double>();
+            block_combine(N, m, kk, Cm.data(), w.data(), (const double *const *)V.data(),
+                          (const double *const *)AV.data(), Rp.data(), rn.data(), st);
+        } else {
+            for (int i = 0; i < kk; ++i)
+                rn[i] = ritz_and_residual(N, m, coef[i].data(), w[i], (const double *const *)V.data(),
+                                          (const double *const *)AV.data(), R[i].as<double>(), st);
         }
+        for (int i = 0; i < std::min(nroots, kk); ++i) rmax = std::max(rmax, rn[i]);
         for (int i = 0; i < nroots; ++i) energies[i] = w[i];  // current estimates
         out.max_residual = rmax;
         if (verbose) {  // progress goes to stderr: stdout belongs to the caller
@@ -234,9 +265,15 @@ DavidsonOut davidson(DevHam *H, int nroots, int mode, int block, double tol, dou
         if (rmax < cur_tol) {
             if (phase == 1) {
                 // the kk lowest Ritz vectors -> new subspace, FP64 sigma from here on
-                for (int i = 0; i < kk; ++i)
-                    ritz_vector(N, m, coef[i].data(), (const double *const *)V.data(),
-                                tmp.as<double>() + (size_t)i * N, st);
+                {
+                    std::vector<double> Cm((size_t)m * kk);
+                    std::vector<double *> Tp(kk);
+                    for (int j = 0; j < m; ++j)
+                        for (int i = 0; i < kk; ++i) Cm[(size_t)j * kk + i] = coef[i][j];
+                    for (int i = 0; i < kk; ++i) Tp[i] = tmp.as<double>() + (size_t)i * N;
+                    block_combine(N, m, kk, Cm.data(), nullptr, (const double *const *)V.data(), nullptr,
+                                  Tp.data(), nullptr, st);
+                }
                 for (int i = 0; i < kk; ++i)
                     HIPOTF_CHECK(hipMemcpyAsync(V[i], tmp.as<double>() + (size_t)i * N, N * sizeof(double),
                                                 hipMemcpyDeviceToDevice, st));
@@ -271,9 +308,24 @@ DavidsonOut davidson(DevHam *H, int nroots, int mode, int block, double tol, dou
             for (int i = 0; i < keep; ++i)
                 for (int j = 0; j < m; ++j) ck[i][j] = C[(size_t)j * m + i];
             DevBuf nv((size_t)keep * N * sizeof(double)), nav((size_t)keep * N * sizeof(double));
-            for (int i = 0; i < keep; ++i) {
-                ritz_vector(N, m, ck[i].data(), (const double *const *)V.data(), nv.as<double>() + (size_t)i * N, st);
-                ritz_vector(N, m, ck[i].data(), (const double *const *)AV.data(), nav.as<double>() + (size_t)i * N, st);
+            if (mode == 1) {
+                std::vector<double> Cm((size_t)m * keep);
+                std::vector<double *> P1(keep), P2(keep);
+                for (int j = 0; j < m; ++j)
+                    for (int i = 0; i < keep; ++i) Cm[(size_t)j * keep + i] = ck[i][j];
+                for (int i = 0; i < keep; ++i) {
+                    P1[i] = nv.as<double>() + (size_t)i * N;
+                    P2[i] = nav.as<double>() + (size_t)i * N;
+                }
+                block_combine(N, m, keep, Cm.data(), nullptr, (const double *const *)V.data(), nullptr, P1.data(),
+                              nullptr, st);
+                block_combine(N, m, keep, Cm.data(), nullptr, (const double *const *)AV.data(), nullptr,
+                              P2.data(), nullptr, st);
+            } else {
+                for (int i = 0; i < keep; ++i) {
+                    ritz_vector(N, m, ck[i].data(), (const double *const *)V.data(), nv.as<double>() + (size_t)i * N, st);
+                    ritz_vector(N, m, ck[i].data(), (const double *const *)AV.data(), nav.as<double>() + (size_t)i * N, st);
+                }
             }
             for (int i = 0; i < keep; ++i) {
                 HIPOTF_CHECK(hipMemcpyAsync(V[i], nv.as<double>() + (size_t)i * N, N * sizeof(double),
@@ -286,7 +338,39 @@ DavidsonOut davidson(DevHam *H, int nroots, int mode, int block, double tol, dou
             set_cols(0, m);
         }
         const int first = m;
-        for (int i : todo) {
+        if (mode == 1) {
+            // preconditioned residuals of up to 8 pairs at once, orthogonalised against the
+            // subspace by two passes of block classical Gram-Schmidt, then against each other
+            int b = 0;
+            std::vector<double> nb0;
+            for (int i : todo) {
+                if (m + b >= maxspace || b >= 8) break;
+                double *t = V[m + b];
+                HIPOTF_CHECK(hipMemcpyAsync(t, R[i].p, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+                precondition(N, t, diag.as<double>(), w[i], std::max(rn[i] * 1e-2, 1e-4), st);
+                nb0.push_back(std::sqrt(dot(N, t, t, st)));
+                ++b;
+            }
+            if (b > 0 && m > 0) {
+                std::vector<double> G((size_t)m * b);
+                for (int pass = 0; pass < 2; ++pass) {
+                    block_dots(N, m, (const double *const *)V.data(), b, (const double *const *)&V[m], G.data(), st);
+                    block_subtract(N, m, (const double *const *)V.data(), G.data(), b, &V[m], st);
+                }
+            }
+            int kept = 0;
+            for (int j = 0; j < b; ++j) {
+                std::swap(V[m + kept], V[m + j]);  // candidate into the next free slot
+                double *t = V[m + kept];
+                double na = mgs(N, kept, (const double *const *)&V[m], t, st);
+                if (kept > 0 && na < 0.717 * nb0[j]) na = mgs(N, kept, (const double *const *)&V[m], t, st);
+                if (na < 1e-12 * std::max(1.0, nb0[j])) continue;  // lies in the subspace
+                normalize(N, t, st);
+                ++kept;
+            }
+            m += kept;
+        }
+        for (int i : (mode == 1 ? std::vector<int>() : todo)) {
             if (m >= maxspace) break;
             const double shift = std::max(rn[i] * 1e-2, 1e-4);
             double *t = V[m];

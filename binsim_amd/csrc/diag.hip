@@ -241,4 +241,240 @@ void multidot(i64 N, int dim, const double *const *V, const double *a, double *o
     std::copy(res.begin(), res.end(), out_host);
 }
 
+// ------------------------------------------------------------------------------------------
+// Block subspace operations of the mixed-precision block Davidson (no reference counterpart:
+// jl/davidson.jl works one vector at a time).  Each reads every subspace vector once per call
+// instead of once per (root, vector) pair; reductions stay two-stage and deterministic.
+// ------------------------------------------------------------------------------------------
+constexpr int BK = 8;  // outputs per launch
+
+struct Ptr64 { const double *p[DMAX]; };
+struct Out8 { double *p[BK]; };
+
+// out_r = sum_j C[j*ldc + r] X_j  (r < nout <= BK); with Y: out_r = sum_j C[j*ldc+r] (Y_j - w_r X_j)
+// and partial[r * nblk + b] = block sums of out_r^2
+template <int KO>
+__global__ __launch_bounds__(256) void k_block_combo(Ptr64 X, Ptr64 Y, int m, int has_y, const double *__restrict__ C,
+                                                     int ldc, const double *__restrict__ w, Out8 out,
+                                                     double *__restrict__ partial, i64 n) {
+    double nrm[KO];
+#pragma unroll
+    for (int r = 0; r < KO; ++r) nrm[r] = 0.0;
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
+        double x[KO], y[KO];
+#pragma unroll
+        for (int r = 0; r < KO; ++r) x[r] = y[r] = 0.0;
+        for (int j = 0; j < m; ++j) {
+            const double xv = X.p[j][i];
+            const double yv = has_y ? Y.p[j][i] : 0.0;
+#pragma unroll
+            for (int r = 0; r < KO; ++r) {
+                const double c = C[j * ldc + r];
+                x[r] += c * xv;
+                y[r] += c * yv;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < KO; ++r) {
+            const double o = has_y ? y[r] - w[r] * x[r] : x[r];
+            out.p[r][i] = o;
+            nrm[r] += o * o;
+        }
+    }
+    if (partial) {
+#pragma unroll
+        for (int r = 0; r < KO; ++r) {
+            const double t = block_sum_d(nrm[r]);
+            if (threadIdx.x == 0) partial[r * gridDim.x + blockIdx.x] = t;
+        }
+    }
+}
+
+// T_j -= sum_i P[i*ldp + j] V_i   (j < nt <= BK, i < m <= DMAX)
+template <int KO>
+__global__ __launch_bounds__(256) void k_block_subtract(Ptr64 V, int m, const double *__restrict__ P, int ldp,
+                                                        Out8 T, i64 n) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
+        double acc[KO];
+#pragma unroll
+        for (int r = 0; r < KO; ++r) acc[r] = 0.0;
+        for (int j = 0; j < m; ++j) {
+            const double v = V.p[j][i];
+#pragma unroll
+            for (int r = 0; r < KO; ++r) acc[r] += P[j * ldp + r] * v;
+        }
+#pragma unroll
+        for (int r = 0; r < KO; ++r) T.p[r][i] -= acc[r];
+    }
+}
+
+// partial[(a * nb + b) * nblk + blk] = block sums of A_a . B_b   (a < na <= 4, b < nb <= BK)
+template <int NA>
+__global__ __launch_bounds__(256) void k_block_dots(Ptr64 A, int na, Ptr64 B, int nb, double *__restrict__ partial,
+                                                    i64 n) {
+    double acc[NA][BK];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int b = 0; b < BK; ++b) acc[a][b] = 0.0;
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
+        double bv[BK];
+#pragma unroll
+        for (int b = 0; b < BK; ++b) bv[b] = b < nb ? B.p[b][i] : 0.0;
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            if (a < na) {
+                const double av = A.p[a][i];
+#pragma unroll
+                for (int b = 0; b < BK; ++b) acc[a][b] += av * bv[b];
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int b = 0; b < BK; ++b) {
+            if (a < na && b < nb) {
+                const double t = block_sum_d(acc[a][b]);
+                if (threadIdx.x == 0) partial[(a * nb + b) * gridDim.x + blockIdx.x] = t;
+            }
+        }
+}
+
+// out[o] = sum of partial[o * nparts .. + nparts) for o = blockIdx.x
+__global__ __launch_bounds__(256) void k_multi_reduce(const double *__restrict__ partial, int nparts,
+                                                      double *__restrict__ out) {
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) acc += partial[(i64)blockIdx.x * nparts + i];
+    const double t = block_sum_d(acc);
+    if (threadIdx.x == 0) out[blockIdx.x] = t;
+}
+
+struct BlockScratch {
+    DevBuf partial, coef, small;
+    BlockScratch() {
+        partial.alloc((size_t)4 * BK * 512 * sizeof(double));
+        coef.alloc((size_t)DMAX * DMAX * sizeof(double));
+        small.alloc((size_t)4 * DMAX * BK * sizeof(double));
+    }
+};
+static BlockScratch &bscratch() {
+    static thread_local std::unique_ptr<BlockScratch> s;
+    if (!s) s.reset(new BlockScratch());
+    return *s;
+}
+static int nblk_block(i64 n) { return (int)std::max<i64>(1, std::min<i64>((n + 255) / 256, 512)); }
+
+template <int KO>
+static void launch_combo(i64 N, const double *const *X, const double *const *Y, int m, const double *Cdev, int ldc,
+                         const double *wdev, double *const *out, double *partial, hipStream_t st) {
+    Ptr64 PX{}, PY{};
+    for (int j = 0; j < m; ++j) {
+        PX.p[j] = X[j];
+        PY.p[j] = Y ? Y[j] : nullptr;
+    }
+    Out8 O{};
+    for (int r = 0; r < KO; ++r) O.p[r] = out[r];
+    hipLaunchKernelGGL(k_block_combo<KO>, dim3(nblk_block(N)), dim3(256), 0, st, PX, PY, m, Y ? 1 : 0, Cdev, ldc,
+                       wdev, O, partial, N);
+    HIPOTF_CHECK(hipGetLastError());
+}
+
+template <typename F>
+static void dispatch_k(int k, F &&f) {
+    switch (k) {
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 3: f(std::integral_constant<int, 3>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 5: f(std::integral_constant<int, 5>{}); break;
+    case 6: f(std::integral_constant<int, 6>{}); break;
+    case 7: f(std::integral_constant<int, 7>{}); break;
+    default: f(std::integral_constant<int, 8>{}); break;
+    }
+}
+
+// out_r = sum_j C[j][r] X_j (Y == nullptr) or residuals sum_j C[j][r] (Y_j - w_r X_j) with their
+// norms in norms[r] (host); C is host row-major [m][k]; k may exceed BK (done in slices)
+void block_combine(i64 N, int m, int k, const double *C, const double *w, const double *const *X,
+                   const double *const *Y, double *const *out, double *norms, hipStream_t st) {
+    HIPOTF_REQUIRE(m >= 1 && m <= DMAX && k >= 1 && k <= DMAX, "block_combine: bad sizes");
+    auto &S = bscratch();
+    std::vector<double> cw((size_t)m * BK + BK);
+    const int nb = nblk_block(N);
+    for (int r0 = 0; r0 < k; r0 += BK) {
+        const int kk = std::min(BK, k - r0);
+        for (int j = 0; j < m; ++j)
+            for (int r = 0; r < BK; ++r) cw[(size_t)j * BK + r] = r < kk ? C[(size_t)j * k + r0 + r] : 0.0;
+        for (int r = 0; r < BK; ++r) cw[(size_t)m * BK + r] = (w && r < kk) ? w[r0 + r] : 0.0;
+        HIPOTF_CHECK(hipMemcpyAsync(S.coef.p, cw.data(), cw.size() * sizeof(double), hipMemcpyHostToDevice, st));
+        const double *Cd = S.coef.as<double>();
+        const double *wd = Cd + (size_t)m * BK;
+        double *part = norms ? S.partial.as<double>() : nullptr;
+        dispatch_k(kk, [&](auto kc) {
+            constexpr int KO = decltype(kc)::value;
+            launch_combo<KO>(N, X, Y, m, Cd, BK, wd, out + r0, part, st);
+        });
+        if (norms) {
+            hipLaunchKernelGGL(k_multi_reduce, dim3(kk), dim3(256), 0, st, S.partial.as<double>(), nb,
+                               S.small.as<double>());
+            HIPOTF_CHECK(hipGetLastError());
+            std::vector<double> h(kk);
+            HIPOTF_CHECK(hipMemcpyAsync(h.data(), S.small.p, kk * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIPOTF_CHECK(hipStreamSynchronize(st));
+            for (int r = 0; r < kk; ++r) norms[r0 + r] = std::sqrt(h[r]);
+        }
+        // the next slice rewrites the coefficient buffer: let this one finish reading it
+        HIPOTF_CHECK(hipStreamSynchronize(st));
+    }
+}
+
+// G[a][b] = A_a . B_b into host row-major [na][nb] (na <= DMAX, nb <= BK)
+void block_dots(i64 N, int na, const double *const *A, int nbv, const double *const *B, double *G,
+                hipStream_t st) {
+    HIPOTF_REQUIRE(na >= 0 && na <= DMAX && nbv >= 1 && nbv <= BK, "block_dots: bad sizes");
+    auto &S = bscratch();
+    const int nb = nblk_block(N);
+    Ptr64 PB{};
+    for (int b = 0; b < nbv; ++b) PB.p[b] = B[b];
+    std::vector<double> h((size_t)4 * BK);
+    for (int a0 = 0; a0 < na; a0 += 4) {
+        const int ka = std::min(4, na - a0);
+        Ptr64 PA{};
+        for (int a = 0; a < ka; ++a) PA.p[a] = A[a0 + a];
+        hipLaunchKernelGGL(k_block_dots<4>, dim3(nb), dim3(256), 0, st, PA, ka, PB, nbv, S.partial.as<double>(), N);
+        HIPOTF_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_multi_reduce, dim3(ka * nbv), dim3(256), 0, st, S.partial.as<double>(), nb,
+                           S.small.as<double>());
+        HIPOTF_CHECK(hipGetLastError());
+        HIPOTF_CHECK(hipMemcpyAsync(h.data(), S.small.p, (size_t)ka * nbv * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPOTF_CHECK(hipStreamSynchronize(st));
+        for (int a = 0; a < ka; ++a)
+            for (int b = 0; b < nbv; ++b) G[(size_t)(a0 + a) * nbv + b] = h[(size_t)a * nbv + b];
+    }
+}
+
+// T_j -= sum_i P[i][j] V_i  (P host row-major [m][nt], nt <= BK)
+void block_subtract(i64 N, int m, const double *const *V, const double *P, int nt, double *const *T,
+                    hipStream_t st) {
+    HIPOTF_REQUIRE(m >= 0 && m <= DMAX && nt >= 1 && nt <= BK, "block_subtract: bad sizes");
+    if (m == 0) return;
+    auto &S = bscratch();
+    std::vector<double> pw((size_t)m * BK, 0.0);
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < nt; ++j) pw[(size_t)i * BK + j] = P[(size_t)i * nt + j];
+    HIPOTF_CHECK(hipMemcpyAsync(S.coef.p, pw.data(), pw.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    Ptr64 PV{};
+    for (int i = 0; i < m; ++i) PV.p[i] = V[i];
+    Out8 O{};
+    for (int j = 0; j < nt; ++j) O.p[j] = T[j];
+    dispatch_k(nt, [&](auto kc) {
+        constexpr int KO = decltype(kc)::value;
+        hipLaunchKernelGGL(k_block_subtract<KO>, dim3(nblk_block(N)), dim3(256), 0, st, PV, m,
+                           S.coef.as<double>(), BK, O, N);
+    });
+    HIPOTF_CHECK(hipGetLastError());
+    HIPOTF_CHECK(hipStreamSynchronize(st));
+}
+
 }  // namespace hipotf

@@ -491,18 +491,28 @@ static bool deterministic() {
     return det;
 }
 
+template <typename T, int NVS, int R>
+static bool try_ab_src_r(DevHam *H, const T *xt, T *s1, int nvst, int v0, const T *eri, hipStream_t st);
+
 template <typename T, int NVS>
 static bool try_ab_src(DevHam *H, const T *xt, T *s1, int nvst, int v0, const T *eri, hipStream_t st) {
     if constexpr (NVS > 2 || (sizeof(T) == 8 && NVS == 2)) {
         return false;  // accumulators would not fit the register file
     } else {
+        // R copies of the integral row: 32 lanes of 4-byte reads are served together, so 32
+        // copies make them conflict-free; 8-byte reads get 16 (LDS space).  (FP32 with 16
+        // copies measured the same at 16 orbitals.)
+        return try_ab_src_r<T, NVS, sizeof(T) == 8 ? 16 : 32>(H, xt, s1, nvst, v0, eri, st);
+    }
+}
+
+template <typename T, int NVS, int R>
+static bool try_ab_src_r(DevHam *H, const T *xt, T *s1, int nvst, int v0, const T *eri, hipStream_t st) {
+    {
         DevBasis *B = H->basis;
         const DevSpin &A = *B->pA, &Bs = *B->pB;
         const int n = H->n, n2 = n * n;
         const int kB = Bs.k, vB = Bs.n - Bs.k;
-        // R copies of the integral row: 32 lanes of 4-byte reads are served together, so 32
-        // copies make them conflict-free; 8-byte reads get 16 (LDS space)
-        constexpr int R = sizeof(T) == 8 ? 16 : 32;
         // threads per source string: 1 (512-thread blocks, 2 waves per SIMD, accumulators in
         // up to 256 VGPRs); HIPOTF_AB_SPLIT = 2 takes 1024-thread blocks with half the slots
         // per thread (measurement)
