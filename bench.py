@@ -187,6 +187,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pmc-unit", action="store_true",
                     help="one FP64 sigma and one plain copy of the same size (counter runs)")
+    ap.add_argument("--tune", action="store_true",
+                    help="run the sigma autotuner on the workload first (about a minute) instead "
+                         "of applying the stored choice")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -198,6 +201,16 @@ def main():
     t = time.perf_counter()
     B, H, _, F = build(n, na, nb)
     setup_s = time.perf_counter() - t
+    from binsim_amd import autotune as AT
+    if args.tune:
+        entry, source = AT.tune(F, B, H, k=4, block_probe=False, path=None, log=log), "tuned now"
+    else:
+        entry = AT.lookup(B)
+        source = "binsim_amd/tuning.json" if entry is not None else "library defaults"
+        if entry is not None:
+            AT.apply(entry)
+    tuning = {"source": source, "fp64": AT.get_config(AT.FP64), "fp32": AT.get_config(AT.FP32)}
+    log(f"launch choices ({source}): {tuning}")
     x = seeded(torch, B.dim, 1000 + 16)
     y = torch.empty_like(x)
     if args.pmc_unit:
@@ -238,6 +251,7 @@ def main():
                      "kernel": "one sigma build (hvec_cuda): k_gauge_in, 2 x k_rowspmm, k_ab_src, "
                                "k_finalize on the default stream, timed with HIP events as one unit"},
         "sigma_fp32_block4_ms_per_vector": round(ms32, 3),
+        "tuning": tuning,
         "setup_s": round(setup_s, 3), "wall_s_timed": round(wall, 3),
     }
     if not args.quick:
@@ -247,6 +261,9 @@ def main():
         torch.cuda.empty_cache()
         from binsim_amd import davidson as D
         B0, H0, _, _ = build(12, 6, 6)
+        e12 = AT.lookup(B0)
+        for prec in (AT.FP64, AT.FP32):
+            AT.set_config(prec, e12["fp64" if prec == AT.FP64 else "fp32"]["config"] if e12 else {})
         hf = torch.zeros(B0.dim, dtype=torch.float64, device="cuda")
         hf[0] = 1.0
         e, _, info = D.solve(B0, H0, nroots=1, mixed=False, tol=1e-8, guess=hf)

@@ -169,6 +169,20 @@ void integrals_from_reference_layout(int n, const double *h_colmajor, const doub
 DevHam *make_dev_ham(DevBasis *b, int n, const std::vector<double> &h, const std::vector<double> &eri,
                      double ecore);
 
+// launch choices of the sigma kernels (process-wide; set by hipotf_set_tuning, chosen by the
+// autotuner in binsim_amd/autotune.py)
+struct SigmaTuning {
+    int ab_kernel = 0;      // alpha-beta kernel: 0 first that applies (source-lane, register,
+                            // chunked), 1 source-lane, 2 register-descriptor, 3 chunked
+    int ab_split = 1;       // source-lane: threads per source string (1: 8 waves/CU, 2: 16)
+    int ab_db = 1;          // source-lane: integral-row buffers (1, or 2 where LDS allows)
+    int ab_nvs = 2;         // source-lane, FP32: vectors per launch (1 or 2)
+    int rs_passes = 1;      // row SpMM: launches over source ranges (1, 2, 4, 8)
+    int deterministic = 0;  // 1: alpha-beta scatter wave by wave (bitwise reproducible)
+};
+SigmaTuning &tuning(int precision);  // 0: FP64 kernels, 1: FP32 kernels
+constexpr int TUNING_FIELDS = 6;
+
 // sigma: dst_v = H src_v for nv vectors; prec 0 = float64 work, 1 = float32 work
 void sigma(DevHam *H, int nv, const double *const *src, double *const *dst, int prec,
            hipStream_t st);

@@ -434,12 +434,9 @@ static void launch_rowspmm(const T *X, T *Y, const SameSpinTable &tab, const T *
     // passes over source ranges (each pass's source columns of a tile span 1/npass of the
     // rows).  Measured at 16 orbitals: 1, 2 and 4 passes take the same time -- the gathers
     // are bound by the per-CU load pipeline (~31 B/clk), not by L2 misses -- so the default
-    // is one pass; HIPOTF_RS_PASSES = 2|4|8 (measurement)
-    static const int passes_env = [] {
-        const char *s = std::getenv("HIPOTF_RS_PASSES");
-        return s ? std::atoi(s) : 0;
-    }();
-    const int npass = passes_env == 2 || passes_env == 4 || passes_env == 8 ? passes_env : 1;
+    // is one pass
+    const int tp = tuning(sizeof(T) == 4).rs_passes;
+    const int npass = tp == 2 || tp == 4 || tp == 8 ? tp : 1;
     for (int p = 0; p < npass; ++p)
         hipLaunchKernelGGL(k_rowspmm<T>, dim3((unsigned)nblocks), dim3(256), 0, st, X, Y,
                            tab.J.as<int32_t>(), tab.P.as<int32_t>(), C, tab.L, nrows, ncols, ntiles,
@@ -470,26 +467,28 @@ static void launch_ab_ept(DevHam *H, const T *xt, T *s1, int nvst, int v0, const
 template <typename T, int NVS>
 constexpr int ab_eptmax() { return (sizeof(T) == 8 && NVS >= 4) ? 4 : 8; }
 
-// alpha-beta kernel family: HIPOTF_AB_KERNEL = "src" | "reg" | "chunk" restricts the choice
-// (for measurement); by default the first one that applies is taken in that order
-static int ab_kernel_pref() {
-    static int pref = -1;
-    if (pref < 0) {
-        const char *s = std::getenv("HIPOTF_AB_KERNEL");
-        pref = !s ? 0 : !std::strcmp(s, "src") ? 1 : !std::strcmp(s, "reg") ? 2 : !std::strcmp(s, "chunk") ? 3 : 0;
-    }
-    return pref;
+// alpha-beta kernel family (SigmaTuning::ab_kernel): 0 takes the first of source-lane,
+// register, chunked that applies; 1..3 start the search at that one
+static int ab_kernel_pref(int prec) {
+    const int p = tuning(prec).ab_kernel;
+    return p >= 0 && p <= 3 ? p : 0;
 }
 
-// HIPOTF_DETERMINISTIC=1: bitwise reproducible sigma (the alpha-beta scatter runs wave by
-// wave, ~15% slower); by default its FP summation order depends on scheduling
-static bool deterministic() {
-    static const bool det = [] {
+// bitwise reproducible sigma (the alpha-beta scatter runs wave by wave, ~15% slower); by
+// default its FP summation order depends on scheduling.  The initial value comes from the
+// environment (HIPOTF_DETERMINISTIC=1).
+SigmaTuning &tuning(int precision) {
+    static SigmaTuning t[2] = {[] {
+        SigmaTuning d;
         const char *s = std::getenv("HIPOTF_DETERMINISTIC");
-        return s && std::atoi(s) != 0;
-    }();
-    return det;
+        d.deterministic = s && std::atoi(s) != 0;
+        return d;
+    }()};
+    static bool init = [] { t[1] = t[0]; return true; }();
+    (void)init;
+    return t[precision ? 1 : 0];
 }
+static bool deterministic(int prec) { return tuning(prec).deterministic != 0; }
 
 template <typename T, int NVS, int R>
 static bool try_ab_src_r(DevHam *H, const T *xt, T *s1, int nvst, int v0, const T *eri, hipStream_t st);
@@ -514,30 +513,22 @@ static bool try_ab_src_r(DevHam *H, const T *xt, T *s1, int nvst, int v0, const 
         const int n = H->n, n2 = n * n;
         const int kB = Bs.k, vB = Bs.n - Bs.k;
         // threads per source string: 1 (512-thread blocks, 2 waves per SIMD, accumulators in
-        // up to 256 VGPRs); HIPOTF_AB_SPLIT = 2 takes 1024-thread blocks with half the slots
-        // per thread (measurement)
-        static const int split_env = [] {
-            const char *s = std::getenv("HIPOTF_AB_SPLIT");
-            return s ? std::atoi(s) : 0;
-        }();
-        const int split = split_env == 2 ? 2 : 1;
+        // up to 256 VGPRs) or 2 (1024-thread blocks with half the slots per thread)
+        constexpr int prec = sizeof(T) == 4;
+        const int split = tuning(prec).ab_split == 2 ? 2 : 1;
         const i64 NB = Bs.N;
         const int bd = AB_SRC_NJ * split;
         if (n2 > AB_SRC_NWE || A.nsx > AB_SRC_CONN || n * (8 + 2) > AB_SRC_BINOM) return false;
         // a W fill reads AB_SRC_NWE entries from the start of one (pq|..) row
         static_assert(AB_SRC_NWE <= ERI_PAD + 1, "integral tables are padded for one fill past the last row");
         // + the binomial table C(p, j), p < n <= 16, j < KM + 2 <= 10; two replicated rows
-        // (one barrier per connection) when they fit, HIPOTF_AB_DB = 1 forces one (measurement)
+        // (one barrier per connection) if tuned so and they fit
         auto lds_for = [&](int db) {
             return ((size_t)db * AB_SRC_NWE * R + AB_SRC_NWE + 16 + (size_t)NB * NVS) * sizeof(T) +
                    (AB_SRC_BINOM + AB_SRC_CONN) * 4;
         };
-        static const int db_env = [] {
-            const char *s = std::getenv("HIPOTF_AB_DB");
-            return s ? std::atoi(s) : 0;
-        }();
         const size_t lds_max = std::min<size_t>(lds_budget(), 160 * 1024);
-        const int db = (db_env != 1 && lds_for(2) <= lds_max) ? 2 : 1;
+        const int db = (tuning(prec).ab_db == 2 && lds_for(2) <= lds_max) ? 2 : 1;
         const size_t smem = lds_for(db);
         if (smem > lds_max) return false;
         const i64 per = (A.N + 7) / 8;
@@ -548,7 +539,7 @@ static bool try_ab_src_r(DevHam *H, const T *xt, T *s1, int nvst, int v0, const 
                                              (int)smem));
             hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(bd), smem, st, xt, s1, nvst, v0,
                                A.sx.as<u32>(), Bs.sx.as<u32>(), A.strs.as<u64>(), Bs.strs.as<u64>(), eri,
-                               n, A.N, NB, A.nsx, kB, vB, deterministic() ? 1 : 0);
+                               n, A.N, NB, A.nsx, kB, vB, deterministic(prec) ? 1 : 0);
             HIPOTF_CHECK(hipGetLastError());
         };
         auto pick = [&](auto split_c, auto db_c) {
@@ -574,7 +565,7 @@ static void launch_ab_nvs(DevHam *H, const T *xt, T *s1, int nvst, int v0, const
     const i64 NB = B->pB->N;
     const int n2 = H->n * H->n;
     // register-descriptor kernel whenever a whole source row fits LDS
-    if (ab_kernel_pref() != 3) {
+    if (ab_kernel_pref(sizeof(T) == 4) != 3) {
         const DevSpin &A = *B->pA, &Bs = *B->pB;
         const size_t wb = (size_t)((n2 + H->n + 15) & ~15) * sizeof(T);
         const size_t smem = wb + (size_t)NB * NVS * sizeof(T);
@@ -626,13 +617,9 @@ static void launch_ab(DevHam *H, const T *xt, T *s1, int nv, const T *eri, hipSt
     const size_t budget = std::min<size_t>(lds_budget(), 160 * 1024) - 4096;
     auto fits = [&](int s) { return (size_t)NB * s * sizeof(T) <= budget; };
     int v0 = 0;
-    if (ab_kernel_pref() <= 1) {
+    if (ab_kernel_pref(sizeof(T) == 4) <= 1) {
         while (v0 < nv) {
-            // HIPOTF_AB_NVS=1: one vector per launch (measurement)
-            static const bool one = [] {
-                const char *s = std::getenv("HIPOTF_AB_NVS");
-                return s && std::atoi(s) == 1;
-            }();
+            const bool one = tuning(sizeof(T) == 4).ab_nvs == 1;
             if (!one && nv - v0 >= 2 && try_ab_src<T, 2>(H, xt, s1, nv, v0, eri, st)) { v0 += 2; continue; }
             if (try_ab_src<T, 1>(H, xt, s1, nv, v0, eri, st)) { v0 += 1; continue; }
             break;

@@ -94,6 +94,15 @@ void *hipotf_build_pool_f64(int64_t norb, int64_t nops, const int32_t *offsets,
 int64_t hipotf_num_generators(void *net);
 /* builds the basis-dependent device tables of an operator now instead of on first use */
 int hipotf_prepare(void *basis, void *net);
+/* launch choices of the sigma kernels, process-wide (no reference counterpart; the
+ * reference's tiles are compile-time, src/include/ham/otf.hpp:6-9 and cuda/common.cuh:7-12):
+ * v[6] = {alpha-beta kernel 0 auto | 1 source-lane | 2 register | 3 chunked, threads per
+ * source string 1|2, integral-row buffers 1|2, FP32 vectors per alpha-beta launch 1|2,
+ * row-SpMM source-range passes 1|2|4|8, deterministic 0|1}, for the FP64 (precision 0) or the
+ * FP32 (precision 1) kernels.  binsim_amd/autotune.py measures the candidates and persists its
+ * choice as JSON.  Return 0 on success. */
+int hipotf_set_tuning(int precision, const int32_t *v, int n);
+int hipotf_get_tuning(int precision, int32_t *v, int n);
 /* block sigma: dst[v] = H src[v], v < nvec (host arrays of device pointers).
  * precision 0: float64 arithmetic (as hvec_cuda); 1: float32 coefficients and integrals with
  * float64 input/output vectors (the mixed-precision Davidson bulk). */
