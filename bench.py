@@ -34,10 +34,10 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 
 # HBM bytes of one FP64 sigma build at 16 orbitals 8a8b, from two rocprofv3 --pmc runs of
-# `bench.py --pmc-unit` (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, summed over the five
-# kernels of one sigma), each scaled by what the same counter reads for a plain copy kernel of
-# 1.325 GB in the same run (profiles/sigma_fp64_16o_pmc.md)
-SIGMA_TRAFFIC_16O = 647.9e9
+# `bench.py --pmc-unit` (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, summed over the
+# dispatches of one sigma), each scaled by what the same counter reads for a plain copy kernel
+# of 1.325 GB in the same run (profiles/sigma_fp64_16o_pmc.md)
+SIGMA_TRAFFIC_16O = 196.5e9
 
 
 def log(*a):
@@ -209,7 +209,8 @@ def main():
         source = "binsim_amd/tuning.json" if entry is not None else "library defaults"
         if entry is not None:
             AT.apply(entry)
-    tuning = {"source": source, "fp64": AT.get_config(AT.FP64), "fp32": AT.get_config(AT.FP32)}
+    tuning = {"source": source, "device": AT.device_key(), "fp64": AT.get_config(AT.FP64),
+              "fp32": AT.get_config(AT.FP32)}
     log(f"launch choices ({source}): {tuning}")
     x = seeded(torch, B.dim, 1000 + 16)
     y = torch.empty_like(x)
@@ -248,8 +249,9 @@ def main():
                      "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4),
                      "traffic": SIGMA_TRAFFIC_16O,
                      "algorithmic_bytes_per_launch": bs,
-                     "kernel": "one sigma build (hvec_cuda): k_gauge_in, 2 x k_rowspmm, k_ab_src, "
-                               "k_finalize on the default stream, timed with HIP events as one unit"},
+                     "kernel": "one sigma build (hvec_cuda): k_gauge_in, 2 row SpMMs (k_rbspmm, "
+                               "source-range passes), k_ab_src, k_finalize on the default stream, "
+                               "timed with HIP events as one unit"},
         "sigma_fp32_block4_ms_per_vector": round(ms32, 3),
         "tuning": tuning,
         "setup_s": round(setup_s, 3), "wall_s_timed": round(wall, 3),

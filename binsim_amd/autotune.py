@@ -14,8 +14,11 @@ Tuned per precision (FP64 sigma of one vector; FP32 block sigma of ``k`` vectors
   (1: 512-thread workgroups, 8 waves per CU; 2: 1024-thread workgroups, 16 waves per CU);
 * ``ab_db``: one or two LDS copies of the replicated integral row (source-lane);
 * ``ab_nvs``: vectors per source-lane launch in FP32 (1 or 2);
+* ``rs_block``: destination alpha (beta) strings per workgroup of the same-spin row SpMM that
+  share each gathered source row (4, 8, 16; 0 = one destination row per wave);
 * ``rs_passes``: the same-spin row SpMM gathers its sources in 1, 2, 4 or 8 alpha-string
-  ranges (source alpha block of N/passes strings per launch);
+  ranges (source alpha block of N/passes strings per launch; the library's default, 0, takes
+  the fewest that keep a column tile's gathered slice within an XCD's 4 MB L2);
 
 and, for the mixed-precision solver, the Davidson block size ``k`` (Ritz pairs refined per
 iteration) by the time to a probe tolerance.  Every candidate's sigma is compared against the
@@ -32,17 +35,17 @@ import numpy as np
 
 from . import lib as L
 
-FIELDS = ("ab_kernel", "ab_split", "ab_db", "ab_nvs", "rs_passes", "deterministic")
+FIELDS = ("ab_kernel", "ab_split", "ab_db", "ab_nvs", "rs_passes", "deterministic", "rs_block")
 FP64, FP32 = 0, 1
 DEFAULT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 
 _RANGES = {"ab_kernel": (0, 1, 2, 3), "ab_split": (1, 2), "ab_db": (1, 2), "ab_nvs": (1, 2),
-           "rs_passes": (1, 2, 4, 8), "deterministic": (0, 1)}
+           "rs_passes": (0, 1, 2, 4, 8), "deterministic": (0, 1), "rs_block": (0, 4, 8, 16)}
 
 
 def validate(cfg: dict) -> dict:
     """Return ``cfg`` completed with defaults; raise ValueError on an unknown or bad field."""
-    out = dict(zip(FIELDS, (0, 1, 1, 2, 1, 0)))
+    out = dict(zip(FIELDS, (0, 1, 1, 2, 0, 0, 8)))
     for k, v in cfg.items():
         if k not in _RANGES:
             raise ValueError(f"unknown tuning field {k!r}")
@@ -123,6 +126,9 @@ def tune_sigma(F, basis, precision: int, k: int = 1, reps: int = 3, warmup: int 
                seed: int = 9001, log=None) -> dict:
     """Time every candidate for ``precision`` on ``k`` seeded vectors and install the fastest.
 
+    The alpha-beta candidates are timed first (same-spin part at its defaults), then every
+    same-spin row block size and pass count with the best alpha-beta choice.
+
     Returns ``{"config": ..., "ms": ..., "timings": [(config, ms), ...], "rejected": [...]}``.
     The tolerance for agreeing with the first candidate is 1e-12 relative in FP64 and 1e-5 in
     FP32 (different accumulation orders of the same terms).
@@ -139,7 +145,7 @@ def tune_sigma(F, basis, precision: int, k: int = 1, reps: int = 3, warmup: int 
     ref, timings, rejected = None, [], []
     try:
         for cfg in candidates(precision, k):
-            c = dict(cfg, rs_passes=1, deterministic=saved["deterministic"])
+            c = dict(cfg, rs_passes=0, rs_block=8, deterministic=saved["deterministic"])
             set_config(precision, c)
             ms = _time_sigma(F, xs, ys, precision, reps, warmup)
             if ref is None:
@@ -151,8 +157,8 @@ def tune_sigma(F, basis, precision: int, k: int = 1, reps: int = 3, warmup: int 
             if log:
                 log(f"  prec {precision} k {k} {c}: {ms:.3f} ms")
         best = min(timings, key=lambda t: t[1])[0]
-        for p in (2, 4, 8):
-            c = dict(best, rs_passes=p)
+        for rb, p in [(rb, p) for rb in (0, 4, 8, 16) for p in (1, 2, 4, 8)]:
+            c = dict(best, rs_block=rb, rs_passes=p)
             set_config(precision, c)
             ms = _time_sigma(F, xs, ys, precision, reps, warmup)
             if _max_rel(ys, ref) > tol:
@@ -213,8 +219,21 @@ def save_file(data: dict, path: str = DEFAULT_PATH) -> None:
 
 
 def lookup(basis, path: str = DEFAULT_PATH) -> dict | None:
-    """The stored entry for this device and system, or None."""
-    return load_file(path).get(device_key(), {}).get(system_key(basis))
+    """The stored entry for this device and system, or None.
+
+    The device part of the key is ``name|arch|cu<count>``; the marketing name can differ
+    between driver installs of the same part, so an entry stored under another name with the
+    same architecture and CU count is accepted.
+    """
+    data = load_file(path)
+    key, sysk = device_key(), system_key(basis)
+    if sysk in data.get(key, {}):
+        return data[key][sysk]
+    tail = key.split("|", 1)[-1]
+    for k, v in data.items():
+        if k.split("|", 1)[-1] == tail and sysk in v:
+            return v[sysk]
+    return None
 
 
 def apply(entry: dict) -> None:

@@ -415,18 +415,20 @@ int hipotf_prepare(void *basis, void *net) {
 }
 
 // launch choices of the sigma kernels (process-wide, per precision 0 FP64 / 1 FP32):
-// v[0..n) = {ab_kernel, ab_split, ab_db, ab_nvs, rs_passes, deterministic}; values out of
-// range are rejected
+// v[0..n) = {ab_kernel, ab_split, ab_db, ab_nvs, rs_passes, deterministic, rs_block}; values
+// out of range are rejected
 int hipotf_set_tuning(int precision, const int32_t *v, int n) {
     API_TRY
     HIPOTF_REQUIRE(precision == 0 || precision == 1, "tuning: precision must be 0 or 1");
-    HIPOTF_REQUIRE(v != nullptr && n == TUNING_FIELDS, "tuning: expected 6 values");
+    HIPOTF_REQUIRE(v != nullptr && n == TUNING_FIELDS, "tuning: expected 7 values");
     HIPOTF_REQUIRE(v[0] >= 0 && v[0] <= 3, "tuning: ab_kernel must be 0..3");
     HIPOTF_REQUIRE(v[1] == 1 || v[1] == 2, "tuning: ab_split must be 1 or 2");
     HIPOTF_REQUIRE(v[2] == 1 || v[2] == 2, "tuning: ab_db must be 1 or 2");
     HIPOTF_REQUIRE(v[3] == 1 || v[3] == 2, "tuning: ab_nvs must be 1 or 2");
-    HIPOTF_REQUIRE(v[4] == 1 || v[4] == 2 || v[4] == 4 || v[4] == 8, "tuning: rs_passes must be 1, 2, 4 or 8");
+    HIPOTF_REQUIRE(v[4] == 0 || v[4] == 1 || v[4] == 2 || v[4] == 4 || v[4] == 8,
+                   "tuning: rs_passes must be 0, 1, 2, 4 or 8");
     HIPOTF_REQUIRE(v[5] == 0 || v[5] == 1, "tuning: deterministic must be 0 or 1");
+    HIPOTF_REQUIRE(v[6] == 0 || v[6] == 4 || v[6] == 8 || v[6] == 16, "tuning: rs_block must be 0, 4, 8 or 16");
     SigmaTuning &t = tuning(precision);
     t.ab_kernel = v[0];
     t.ab_split = v[1];
@@ -434,6 +436,7 @@ int hipotf_set_tuning(int precision, const int32_t *v, int n) {
     t.ab_nvs = v[3];
     t.rs_passes = v[4];
     t.deterministic = v[5];
+    t.rs_block = v[6];
     return 0;
     API_CATCH(1)
 }
@@ -441,7 +444,7 @@ int hipotf_set_tuning(int precision, const int32_t *v, int n) {
 int hipotf_get_tuning(int precision, int32_t *v, int n) {
     API_TRY
     HIPOTF_REQUIRE(precision == 0 || precision == 1, "tuning: precision must be 0 or 1");
-    HIPOTF_REQUIRE(v != nullptr && n == TUNING_FIELDS, "tuning: expected 6 values");
+    HIPOTF_REQUIRE(v != nullptr && n == TUNING_FIELDS, "tuning: expected 7 values");
     const SigmaTuning &t = tuning(precision);
     v[0] = t.ab_kernel;
     v[1] = t.ab_split;
@@ -449,6 +452,7 @@ int hipotf_get_tuning(int precision, int32_t *v, int n) {
     v[3] = t.ab_nvs;
     v[4] = t.rs_passes;
     v[5] = t.deterministic;
+    v[6] = t.rs_block;
     return 0;
     API_CATCH(1)
 }

@@ -125,6 +125,67 @@ static void build_same_spin(SameSpinTable &T, const SpinSpace &S, const std::vec
     std::vector<float> C32(C.begin(), C.end());
     T.c32.upload(C32);
     T.gdiag.upload(gd);
+    T.N = S.N;
+    T.hJ = std::move(J);
+    T.hC = std::move(C);
+}
+
+const RowBlockTable &row_blocks(SameSpinTable &T, int rb) {
+    HIPOTF_REQUIRE(rb == 4 || rb == 8 || rb == 16, "row block size must be 4, 8 or 16");
+    const int slot = rb == 4 ? 0 : (rb == 8 ? 1 : 2);
+    if (T.blocks[slot]) return *T.blocks[slot];
+    const i64 N = T.N;
+    const size_t L = T.L;
+    HIPOTF_REQUIRE(T.hJ.size() == (size_t)N * L, "internal: same-spin host table missing");
+    auto R = std::make_unique<RowBlockTable>();
+    R->rb = rb;
+    R->nblk = (N + rb - 1) / rb;
+    const i64 nblk = R->nblk;
+    // the ascending, duplicate-free union of the sources of rows [g*rb, g*rb + rb)
+    auto block_union = [&](i64 g, std::vector<int32_t> &u) {
+        const i64 r0 = g * rb, r1 = std::min<i64>(N, r0 + rb);
+        u.assign(T.hJ.begin() + r0 * L, T.hJ.begin() + r1 * L);
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+    };
+    std::vector<i64> off(nblk + 1, 0);
+    par_for(nblk, [&](i64 g) {
+        std::vector<int32_t> u;
+        block_union(g, u);
+        off[g + 1] = (i64)u.size();
+    });
+    for (i64 g = 0; g < nblk; ++g) off[g + 1] += off[g];
+    const i64 E = off[nblk];
+    HIPOTF_REQUIRE(E < (1ll << 31), "row-block table too large for 32-bit offsets");
+    R->entries = E;
+    std::vector<int32_t> J((size_t)E);
+    std::vector<double> C((size_t)E * rb, 0.0);
+    std::vector<int32_t> P((size_t)nblk * (SS_RANGES + 1));
+    auto range_start = [&](int g) { return (i64)g * N / SS_RANGES; };
+    par_for(nblk, [&](i64 g) {
+        std::vector<int32_t> u;
+        block_union(g, u);
+        std::copy(u.begin(), u.end(), J.begin() + off[g]);
+        const i64 r0 = g * rb, r1 = std::min<i64>(N, r0 + rb);
+        for (i64 I = r0; I < r1; ++I)
+            for (size_t e = 0; e < L; ++e) {
+                const int32_t j = T.hJ[I * L + e];
+                const size_t pos = std::lower_bound(u.begin(), u.end(), j) - u.begin();
+                C[((size_t)off[g] + pos) * rb + (size_t)(I - r0)] += T.hC[I * L + e];
+            }
+        for (int rg = 0; rg <= SS_RANGES; ++rg) {
+            const size_t pos = rg == SS_RANGES ? u.size()
+                                               : std::lower_bound(u.begin(), u.end(), (int32_t)range_start(rg)) - u.begin();
+            P[(size_t)g * (SS_RANGES + 1) + rg] = (int32_t)(off[g] + (i64)pos);
+        }
+    });
+    R->J.upload(J);
+    R->P.upload(P);
+    R->c64.upload(C);
+    std::vector<float> C32(C.begin(), C.end());
+    R->c32.upload(C32);
+    T.blocks[slot] = std::move(R);
+    return *T.blocks[slot];
 }
 
 void integrals_from_reference_layout(int n, const double *h_cm, const double *g_ref,
@@ -187,6 +248,14 @@ DevHam *make_dev_ham(DevBasis *b, int n, const std::vector<double> &h, const std
     } else {
         build_same_spin(H->tB, hb.B, H->h, H->eri, n);
         H->ptB = &H->tB;
+    }
+    // the default row blocks now, so that the first sigma does not pay for them
+    for (int prec = 0; prec < 2; ++prec) {
+        const int rb = tuning(prec).rs_block;
+        if (rb == 4 || rb == 8 || rb == 16) {
+            row_blocks(*H->ptA, rb);
+            row_blocks(*H->ptB, rb);
+        }
     }
     std::vector<double> jab((size_t)hb.A.N * n, 0.0);
     for (i64 a = 0; a < hb.A.N; ++a) {

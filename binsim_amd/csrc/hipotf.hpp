@@ -142,14 +142,34 @@ DevBasis *make_dev_basis(const HostBasis &hb);
 // ------------------------------------------------------------------------------------------
 constexpr int SS_RANGES = 8;  // source-index ranges the same-spin rows are split into
 
+// the same-spin rows grouped in blocks of rb consecutive destination strings: per block the
+// ascending union of its rows' sources, each with rb coefficients (0 where a row of the block
+// has no such source), so that one gathered source row serves rb destination rows
+struct RowBlockTable {
+    int rb = 0;
+    i64 nblk = 0;             // ceil(N / rb)
+    i64 entries = 0;
+    DevBuf J;                 // int32[entries]: sources of block 0, then block 1, ...
+    DevBuf P;                 // int32[nblk][SS_RANGES + 1]: entry offset (absolute) of the
+                              // first source >= g * N / SS_RANGES; P[.][SS_RANGES] = block end
+    DevBuf c64, c32;          // double / float [entries][rb]
+};
+
 struct SameSpinTable {
+    i64 N = 0;
     int L = 0;                // 1 + nsx + ndx entries per dest string, ascending source index
     DevBuf J;                 // int32[N][L]
     DevBuf P;                 // int32[N][SS_RANGES + 1]: first entry of each source range
                               // g (sources >= g * N / SS_RANGES); P[.][SS_RANGES] = L
     DevBuf c64, c32;          // double / float [N][L]
     DevBuf gdiag;             // double[N]  same-spin diagonal G[I,I]
+    std::vector<int32_t> hJ;  // host copies of J and the FP64 coefficients (row blocks are
+    std::vector<double> hC;   // built from them on first use)
+    std::unique_ptr<RowBlockTable> blocks[3];  // rb = 4, 8, 16
 };
+
+// row-block table of T for rb in {4, 8, 16}, built (and uploaded) on first use
+const RowBlockTable &row_blocks(SameSpinTable &T, int rb);
 
 constexpr int ERI_PAD = 256;  // zero entries appended to the device integral tables
 
@@ -177,11 +197,14 @@ struct SigmaTuning {
     int ab_split = 1;       // source-lane: threads per source string (1: 8 waves/CU, 2: 16)
     int ab_db = 1;          // source-lane: integral-row buffers (1, or 2 where LDS allows)
     int ab_nvs = 2;         // source-lane, FP32: vectors per launch (1 or 2)
-    int rs_passes = 1;      // row SpMM: launches over source ranges (1, 2, 4, 8)
+    int rs_passes = 0;      // row SpMM: launches over source ranges (1, 2, 4, 8; 0 = fewest
+                            // for which a column tile's gathered slice fits an XCD's L2)
     int deterministic = 0;  // 1: alpha-beta scatter wave by wave (bitwise reproducible)
+    int rs_block = 8;       // row SpMM: destination rows sharing each gathered source row
+                            // (4, 8, 16), or 0 for one row per wave
 };
 SigmaTuning &tuning(int precision);  // 0: FP64 kernels, 1: FP32 kernels
-constexpr int TUNING_FIELDS = 6;
+constexpr int TUNING_FIELDS = 7;
 
 // sigma: dst_v = H src_v for nv vectors; prec 0 = float64 work, 1 = float32 work
 void sigma(DevHam *H, int nv, const double *const *src, double *const *dst, int prec,

@@ -125,6 +125,66 @@ __global__ __launch_bounds__(256) void k_rowspmm(const T *__restrict__ X, T *__r
 }
 
 // ---------------------------------------------------------------------------------------
+// K2/K3 on row blocks: Y[I0 + r, :] (+)= sum_e C[e][r] X[J[e], :], r < RB, over the union J of
+// the sources of the RB consecutive destination rows I0 .. I0 + RB - 1 (RowBlockTable).  Each
+// gathered source slice feeds RB accumulators, so the gather volume drops to the union's size
+// (at 16 orbitals, 8 alpha: 0.43 of the per-row volume for RB = 8) at the price of RB FMAs per
+// gather, zeros included.  Workgroup = one row block x 256 consecutive columns (4 waves x 64);
+// the source index and the RB coefficients of an entry are uniform (scalar loads) and shared
+// by the 4 waves.  Work items (tile-major: all row blocks of a column tile, then the next
+// tile) are dealt to the 8 XCDs in contiguous ranges, so each XCD sweeps whole column tiles.
+// ---------------------------------------------------------------------------------------
+template <typename T, int RB>
+__global__ __launch_bounds__(256) void k_rbspmm(const T *__restrict__ X, T *__restrict__ Y,
+                                                const int32_t *__restrict__ J,
+                                                const int32_t *__restrict__ P,
+                                                const T *__restrict__ C, i64 nrows, i64 ncols,
+                                                i64 nblk, i64 nwork, i64 per_xcd, int g0, int g1,
+                                                int accumulate) {
+    const i64 b = blockIdx.x;
+    const i64 w = (b & 7) * per_xcd + (b >> 3);
+    if (w >= nwork) return;
+    const i64 tile = w / nblk, blk = w - (w / nblk) * nblk;
+    const i64 col = tile * 256 + threadIdx.x;
+    if (col >= ncols) return;
+    const int eb = P[blk * (SS_RANGES + 1) + g0], ee = P[blk * (SS_RANGES + 1) + g1];
+    T acc[RB];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) acc[r] = 0;
+    const T *Xc = X + col;
+    // batches of U gathers in flight per wave.  (Measured at 16 orbitals: issuing batch i + 1
+    // before batch i's FMAs, or buffer loads with a scalar row base, are 2-7% slower: the
+    // limit is the memory system under load, not the wave's own latency.)
+    constexpr int U = 8;
+    int e = eb;
+    for (; e + U <= ee; e += U) {
+        T xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = Xc[(i64)J[e + u] * ncols];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const T *c = C + (i64)(e + u) * RB;
+#pragma unroll
+            for (int r = 0; r < RB; ++r) acc[r] += c[r] * xv[u];
+        }
+    }
+    for (; e < ee; ++e) {
+        const T x = Xc[(i64)J[e] * ncols];
+        const T *c = C + (i64)e * RB;
+#pragma unroll
+        for (int r = 0; r < RB; ++r) acc[r] += c[r] * x;
+    }
+    const i64 I0 = blk * RB;
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+        if (I0 + r < nrows) {
+            T *y = Y + (I0 + r) * ncols + col;
+            *y = accumulate ? *y + acc[r] : acc[r];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
 // K4: alpha-beta part.  Workgroup = destination alpha row Ia.  For each alpha connection
 // (its nsxA single excitations Ja = Ia - p + q, then Ia itself) the source row x~[Ja, :] is
 // staged in LDS (in chunks if it does not fit) together with the integral row
@@ -422,21 +482,56 @@ static size_t lds_budget() {
     return cached;
 }
 
+template <typename T, int RB>
+static void launch_rbspmm(const T *X, T *Y, const RowBlockTable &rt, i64 nrows, i64 ncols, int npass,
+                          hipStream_t st) {
+    const i64 ntiles = (ncols + 255) / 256;
+    const i64 nwork = ntiles * rt.nblk;
+    const i64 per_xcd = (nwork + 7) / 8;
+    HIPOTF_REQUIRE(per_xcd * 8 < (1ll << 31), "row-block SpMM grid too large");
+    const T *C = sizeof(T) == 4 ? (const T *)rt.c32.p : (const T *)rt.c64.p;
+    for (int p = 0; p < npass; ++p)
+        hipLaunchKernelGGL((k_rbspmm<T, RB>), dim3((unsigned)(per_xcd * 8)), dim3(256), 0, st, X, Y,
+                           rt.J.as<int32_t>(), rt.P.as<int32_t>(), C, nrows, ncols, rt.nblk, nwork,
+                           per_xcd, p * SS_RANGES / npass, (p + 1) * SS_RANGES / npass, p > 0 ? 1 : 0);
+    HIPOTF_CHECK(hipGetLastError());
+}
+
+// source-range passes of a row SpMM: the tuned value, or (0) the fewest passes (1, 2, 4, 8) for
+// which the gathered slice of one column tile, nrows x tile_cols values, fits an XCD's 4 MB L2
 template <typename T>
-static void launch_rowspmm(const T *X, T *Y, const SameSpinTable &tab, const T *C, i64 nrows,
+static int rs_npass(i64 nrows, int tile_cols) {
+    const int tp = tuning(sizeof(T) == 4).rs_passes;
+    if (tp == 1 || tp == 2 || tp == 4 || tp == 8) return tp;
+    const double slice = (double)nrows * tile_cols * sizeof(T), l2 = 4.0 * 1024 * 1024;
+    int p = 1;
+    while (p < 8 && slice > l2 * p) p *= 2;
+    return p;
+}
+
+template <typename T>
+static void launch_rowspmm(const T *X, T *Y, SameSpinTable &tab, const T *C, i64 nrows,
                            i64 ncols, hipStream_t st) {
     if (nrows == 0 || ncols == 0) return;
+    HIPOTF_REQUIRE(nrows == tab.N, "internal: same-spin table does not match the row count");
+    const int rb = tuning(sizeof(T) == 4).rs_block;
+    if (rb == 4 || rb == 8 || rb == 16) {
+        const int npass = rs_npass<T>(nrows, 256);
+        const RowBlockTable &rt = row_blocks(tab, rb);
+        if (rb == 4) launch_rbspmm<T, 4>(X, Y, rt, nrows, ncols, npass, st);
+        else if (rb == 8) launch_rbspmm<T, 8>(X, Y, rt, nrows, ncols, npass, st);
+        else launch_rbspmm<T, 16>(X, Y, rt, nrows, ncols, npass, st);
+        return;
+    }
     const i64 ntiles = (ncols + 63) / 64;
     const i64 bpt = (nrows + 3) / 4;
     const i64 ntiles_pad = (ntiles + 7) / 8 * 8;
     const i64 nblocks = ntiles_pad * bpt;
     HIPOTF_REQUIRE(nblocks < (1ll << 31), "row-SpMM grid too large");
     // passes over source ranges (each pass's source columns of a tile span 1/npass of the
-    // rows).  Measured at 16 orbitals: 1, 2 and 4 passes take the same time -- the gathers
-    // are bound by the per-CU load pipeline (~31 B/clk), not by L2 misses -- so the default
-    // is one pass
-    const int tp = tuning(sizeof(T) == 4).rs_passes;
-    const int npass = tp == 2 || tp == 4 || tp == 8 ? tp : 1;
+    // rows).  Measured at 16 orbitals: 1, 2 and 4 passes take the same time here -- these
+    // gathers are bound by the per-CU load pipeline (~31 B/clk), not by L2 misses
+    const int npass = rs_npass<T>(nrows, 64);
     for (int p = 0; p < npass; ++p)
         hipLaunchKernelGGL(k_rowspmm<T>, dim3((unsigned)nblocks), dim3(256), 0, st, X, Y,
                            tab.J.as<int32_t>(), tab.P.as<int32_t>(), C, tab.L, nrows, ncols, ntiles,
