@@ -36,7 +36,8 @@ template <typename T, int NVS, int KM, int VM, int R, int SPLIT, int DB>
 __global__ __launch_bounds__(AB_SRC_NJ *SPLIT) __attribute__((amdgpu_waves_per_eu(2 * SPLIT, 2 * SPLIT)))
 void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, const u32 *__restrict__ sxA,
               const u32 *__restrict__ sxB, const u64 *__restrict__ strsA, const u64 *__restrict__ strsB,
-              const T *__restrict__ eri, int n, i64 NA, i64 NB, int nsxA, int kB, int vB, int ordered) {
+              const T *__restrict__ eri, const T *__restrict__ dw, int n, i64 NA, i64 NB, int nsxA, int kB,
+              int vB, int ordered) {
     constexpr int BD = AB_SRC_NJ * SPLIT;
     constexpr int KH = KM / SPLIT;                 // occupied-orbital slots per thread
     constexpr int NFM = AB_SRC_NWE * R / BD;       // W values each thread fills
@@ -47,8 +48,8 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
     // DB = 2: two replicated rows, the next connection's filled while the current one is read
     // (one barrier per connection instead of two)
     T *Wr = reinterpret_cast<T *>(smem_raw);       // DB x AB_SRC_NWE * R replicated integral rows
-    T *Wdg = Wr + DB * AB_SRC_NWE * R;             // AB_SRC_NWE + 16: sum_{p in Ia} (pp|rt), zero padded
-    T *S = Wdg + AB_SRC_NWE + 16;                  // NB * NVS destination row
+    T *Z = Wr + DB * AB_SRC_NWE * R;               // 16 zeros (reads of unused slots)
+    T *S = Z + 16;                                 // NB * NVS destination row
     constexpr int BW = KM + 2;
     int *binom = reinterpret_cast<int *>(S + NB * NVS);  // C(p, j), p < n, j < BW
     // the row's alpha connections (single-excitation table entries of Ia), staged so the round
@@ -61,18 +62,9 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
     const i64 per = (NA + 7) / 8;
     const i64 Ia = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (Ia >= NA) return;
-    const u64 astr = strsA[Ia];
+    (void)strsA;
     for (i64 t = tid; t < NB * NVS; t += BD) S[t] = T(0);
-    for (int t = tid; t < AB_SRC_NWE + 16; t += BD) {
-        T v = 0;
-        u64 m = t < n2 ? astr : 0ull;
-        while (m) {
-            const int p = __builtin_ctzll(m);
-            m &= m - 1;
-            v += eri[(i64)(p * n + p) * n2 + t];
-        }
-        Wdg[t] = v;
-    }
+    if (tid < 16) Z[tid] = T(0);
     for (int t = tid; t < n * BW; t += BD) {
         const int p = t / BW, j = t - p * BW;
         u64 c = j <= p ? 1ull : 0ull;
@@ -88,30 +80,32 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
     const u32 copy_off = (u32)((tid & (R - 1)) * sizeof(T));
     const int e0 = tid / R;  // this thread fills entries e0 + f * (BD / R)
 
+    // Every global load of the round loop is unconditional, with clamped in-bounds indices:
+    // a load issued on one path only makes the compiler wait for ALL outstanding loads
+    // (vmcnt(0)) where the paths meet, which would expose each prefetch's full latency.
+    //
     // W values of connection ca (the sign is applied when they are stored, so the loads stay
-    // in flight through the compute phase).  Entries past n2 run into the next row or the
-    // zero padding after the table and land in Wr slots no lane reads.
+    // in flight through the compute phase): row (pq|..) of a single excitation, or the
+    // diagonal row sum_{p in Ia} (pp|..) from dw.  Entries past n2 run into the next row or
+    // the zero padding after the table and land in Wr slots no lane reads.
     auto load_w = [&](T (&pre)[NFM], bool &neg, int ca) {
-        if (ca < nsxA) {
-            const u32 e = conn[ca];
-            const T *er = eri + (i64)(sx_r(e) * n + sx_s(e)) * n2 + e0;
-            neg = sx_neg(e);
+        const bool single = ca < nsxA;
+        const u32 e = conn[single ? ca : 0];
+        const T *er = single ? eri + (i64)(sx_r(e) * n + sx_s(e)) * n2 + e0 : dw + Ia * AB_SRC_NWE + e0;
+        neg = single && sx_neg(e);
 #pragma unroll
-            for (int f = 0; f < NFM; ++f) pre[f] = er[f * (BD / R)];
-        } else {
-            neg = false;
-#pragma unroll
-            for (int f = 0; f < NFM; ++f) pre[f] = Wdg[e0 + f * (BD / R)];
-        }
+        for (int f = 0; f < NFM; ++f) pre[f] = er[f * (BD / R)];
     };
     // source elements of the round (chunk, connection ca); rounds are tracked by incremental
-    // counters: a 64-bit division here would be expanded into vector code in the FMA phase
+    // counters: a 64-bit division here would be expanded into vector code in the FMA phase.
+    // Lanes past the last source string (never scattered) and rounds past the last one
+    // (never consumed) read a valid element instead.
     auto load_x = [&](T (&xv)[NVS], i64 chunk, int ca) {
-        if (chunk >= nchunks) return;
-        const i64 Jb = chunk * AB_SRC_NJ + jl;
-        const i64 Ja = ca < nsxA ? (i64)sx_J(conn[ca]) : Ia;
+        i64 Jb = (chunk < nchunks ? chunk : nchunks - 1) * AB_SRC_NJ + jl;
+        Jb = Jb < NB ? Jb : NB - 1;
+        const i64 Ja = ca < nsxA ? (i64)sx_J(conn[ca < nsxA ? ca : 0]) : Ia;
 #pragma unroll
-        for (int v = 0; v < NVS; ++v) xv[v] = Jb < NB ? xt[(Ja * NB + Jb) * nvst + v0 + v] : T(0);
+        for (int v = 0; v < NVS; ++v) xv[v] = xt[(Ja * NB + Jb) * nvst + v0 + v];
     };
 
     T pre[NFM];
@@ -122,10 +116,8 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
         // round 0's row goes into buffer 0 now; pre then carries round 1's
 #pragma unroll
         for (int f = 0; f < NFM; ++f) Wr[tid + f * BD] = pneg ? -pre[f] : pre[f];
-        if (nit > 1) {
-            wca = 1 % nca;
-            load_w(pre, pneg, wca);
-        }
+        wca = 1 % nca;
+        load_w(pre, pneg, wca);
     }
     T xq[XPF][NVS];
     i64 pchunk = 0;  // round whose source elements are loaded next
@@ -148,15 +140,16 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
     i64 Jb = 0;
     bool live = false;
     // slots beyond (kB, vB) of the instantiated KM x VM grid read valid LDS words into
-    // accumulators that are never scattered; their diagonal terms read the zero padding
-    const u32 zero_off = (u32)((DB * AB_SRC_NWE * R + AB_SRC_NWE) * sizeof(T));
+    // accumulators that are never scattered; their diagonal terms read the zeros Z
+    const u32 zero_off = (u32)(DB * AB_SRC_NWE * R * sizeof(T));
     i64 chunk = 0;
     int ca = 0;
     for (i64 it = 0; it < nit; ++it, advance(chunk, ca)) {
         if (ca == 0) {
             Jb = chunk * AB_SRC_NJ + jl;
             live = Jb < NB;
-            const u64 s = live ? strsB[Jb] : 0ull;
+            const u64 sl = strsB[live ? Jb : NB - 1];
+            const u64 s = live ? sl : 0ull;
             u64 mo = s, mv = ~s & ((n < 64) ? ((1ull << n) - 1) : ~0ull);
             for (int i = 0; i < half * KH; ++i) mo &= mo - 1;  // skip the other shares
 #pragma unroll
@@ -191,17 +184,15 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
                 T *Wn = Wr + ((it + 1) & 1) * AB_SRC_NWE * R;
 #pragma unroll
                 for (int f = 0; f < NFM; ++f) Wn[tid + f * BD] = pneg ? -pre[f] : pre[f];
-                if (it + 2 < nit) {
-                    wca = wca + 1 == nca ? 0 : wca + 1;
-                    load_w(pre, pneg, wca);
-                }
             }
+            wca = wca + 1 == nca ? 0 : wca + 1;
+            load_w(pre, pneg, wca);
         } else {
             __syncthreads();  // everyone is done reading the previous W
 #pragma unroll
             for (int f = 0; f < NFM; ++f) Wr[tid + f * BD] = pneg ? -pre[f] : pre[f];
             __syncthreads();
-            if (it + 1 < nit) load_w(pre, pneg, ca + 1 == nca ? 0 : ca + 1);
+            load_w(pre, pneg, ca + 1 == nca ? 0 : ca + 1);
         }
         T xnew[NVS];
 #pragma unroll

@@ -180,6 +180,8 @@ struct DevHam {
     std::vector<double> h, eri;   // host copies: h[p*n+q], eri[((p*n+q)*n+r)*n+s] = (pq|rs)
     DevBuf eri64, eri32;          // (pq|rs) on device
     DevBuf jab;                   // double[Na][n]: sum_{p in a} (pp|rr)
+    DevBuf dw64, dw32;            // [Na][256]: sum_{p in a} (pp|rt) at r*n + t (n*n <= 256),
+                                  // zero beyond n*n; built on first use by the alpha-beta kernel
     SameSpinTable tA, tB;
     SameSpinTable *ptA = nullptr, *ptB = nullptr;
 };

@@ -588,6 +588,39 @@ static bool deterministic(int prec) { return tuning(prec).deterministic != 0; }
 template <typename T, int NVS, int R>
 static bool try_ab_src_r(DevHam *H, const T *xt, T *s1, int nvst, int v0, const T *eri, hipStream_t st);
 
+// dw[Ia][rt] = sum_{p in Ia} (pp|rt) for rt < n*n, 0 up to AB_SRC_NWE: the integral row of the
+// diagonal alpha connection, read by the source-lane kernel like the (pq|..) rows
+template <typename T>
+__global__ __launch_bounds__(AB_SRC_NWE) void k_diag_w(const u64 *__restrict__ strsA,
+                                                       const double *__restrict__ eri, int n, T *__restrict__ dw) {
+    const i64 Ia = blockIdx.x;
+    const int t = threadIdx.x, n2 = n * n;
+    double v = 0.0;
+    if (t < n2) {
+        u64 m = strsA[Ia];
+        while (m) {
+            const int p = __builtin_ctzll(m);
+            m &= m - 1;
+            v += eri[(i64)(p * n + p) * n2 + t];
+        }
+    }
+    dw[Ia * AB_SRC_NWE + t] = (T)v;
+}
+
+template <typename T>
+static const T *diag_w_table(DevHam *H, hipStream_t st) {
+    DevBuf &b = sizeof(T) == 8 ? H->dw64 : H->dw32;
+    const DevSpin &A = *H->basis->pA;
+    if (!b.p) {
+        HIPOTF_REQUIRE(A.N > 0 && A.N < (1ll << 31), "alpha string count out of range");
+        b.alloc((size_t)A.N * AB_SRC_NWE * sizeof(T));
+        hipLaunchKernelGGL(k_diag_w<T>, dim3((unsigned)A.N), dim3(AB_SRC_NWE), 0, st, A.strs.as<u64>(),
+                           H->eri64.as<double>(), H->n, b.as<T>());
+        HIPOTF_CHECK(hipGetLastError());
+    }
+    return b.as<T>();
+}
+
 template <typename T, int NVS>
 static bool try_ab_src(DevHam *H, const T *xt, T *s1, int nvst, int v0, const T *eri, hipStream_t st) {
     if constexpr (NVS > 2 || (sizeof(T) == 8 && NVS == 2)) {
@@ -619,7 +652,7 @@ static bool try_ab_src_r(DevHam *H, const T *xt, T *s1, int nvst, int v0, const 
         // + the binomial table C(p, j), p < n <= 16, j < KM + 2 <= 10; two replicated rows
         // (one barrier per connection) if tuned so and they fit
         auto lds_for = [&](int db) {
-            return ((size_t)db * AB_SRC_NWE * R + AB_SRC_NWE + 16 + (size_t)NB * NVS) * sizeof(T) +
+            return ((size_t)db * AB_SRC_NWE * R + 16 + (size_t)NB * NVS) * sizeof(T) +
                    (AB_SRC_BINOM + AB_SRC_CONN) * 4;
         };
         const size_t lds_max = std::min<size_t>(lds_budget(), 160 * 1024);
@@ -629,12 +662,13 @@ static bool try_ab_src_r(DevHam *H, const T *xt, T *s1, int nvst, int v0, const 
         const i64 per = (A.N + 7) / 8;
         const i64 nblocks = per * 8;
         HIPOTF_REQUIRE(nblocks < (1ll << 31), "alpha-beta grid too large");
+        const T *dw = diag_w_table<T>(H, st);
         auto go = [&](auto kern) {
             HIPOTF_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                              (int)smem));
             hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(bd), smem, st, xt, s1, nvst, v0,
                                A.sx.as<u32>(), Bs.sx.as<u32>(), A.strs.as<u64>(), Bs.strs.as<u64>(), eri,
-                               n, A.N, NB, A.nsx, kB, vB, deterministic(prec) ? 1 : 0);
+                               dw, n, A.N, NB, A.nsx, kB, vB, deterministic(prec) ? 1 : 0);
             HIPOTF_CHECK(hipGetLastError());
         };
         auto pick = [&](auto split_c, auto db_c) {
