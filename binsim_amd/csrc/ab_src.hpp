@@ -40,9 +40,20 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
               int vB, int ordered) {
     constexpr int BD = AB_SRC_NJ * SPLIT;
     constexpr int KH = KM / SPLIT;                 // occupied-orbital slots per thread
-    constexpr int NFM = AB_SRC_NWE * R / BD;       // W values each thread fills
     constexpr int XPF = AB_SRC_XPF;
-    static_assert(KM % SPLIT == 0 && (AB_SRC_NWE * R) % BD == 0 && BD % R == 0, "fill layout");
+    // W fill: wave w loads entries [w*EPW, (w+1)*EPW) of the row, LPE lanes per entry (one
+    // coalesced load per thread), and each lane stores BPL 16-byte blocks of CB copies of its
+    // entry.  The block order is rotated with the entry so that the 64 lanes of one store
+    // instruction spread evenly over the LDS banks.
+    constexpr int NWV = BD / 64;                   // waves
+    constexpr int EPW = AB_SRC_NWE / NWV;          // entries per wave
+    constexpr int LPE = 64 / EPW;                  // lanes per entry
+    constexpr int CB = 16 / (int)sizeof(T);        // copies per 16-byte block
+    constexpr int NBLK = R / CB;                   // blocks per entry
+    constexpr int BPL = NBLK / LPE;                // blocks stored per lane
+    static_assert(KM % SPLIT == 0 && NWV * EPW == AB_SRC_NWE && LPE * EPW == 64 && NBLK % LPE == 0 &&
+                  BPL >= 1, "fill layout");
+    typedef T TV __attribute__((ext_vector_type(CB)));
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n2 = n * n;
     // DB = 2: two replicated rows, the next connection's filled while the current one is read
@@ -78,7 +89,18 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
     const i64 nchunks = (NB + AB_SRC_NJ - 1) / AB_SRC_NJ;
     const i64 nit = nchunks * nca;
     const u32 copy_off = (u32)((tid & (R - 1)) * sizeof(T));
-    const int e0 = tid / R;  // this thread fills entries e0 + f * (BD / R)
+    const int fe = (tid >> 6) * EPW + (tid & 63) / LPE;  // entry this thread loads and stores
+    u32 foff[BPL];                                        // byte offsets of its blocks in a row
+#pragma unroll
+    for (int q = 0; q < BPL; ++q) {
+        const int b = ((tid & 63) % LPE * BPL + q + (tid & 63) / LPE) % NBLK;
+        foff[q] = (u32)((fe * R + b * CB) * sizeof(T));
+    }
+    auto store_w = [&](T *W, T v) {
+        const TV vv = v;
+#pragma unroll
+        for (int q = 0; q < BPL; ++q) *reinterpret_cast<TV *>(reinterpret_cast<unsigned char *>(W) + foff[q]) = vv;
+    };
 
     // Every global load of the round loop is unconditional, with clamped in-bounds indices:
     // a load issued on one path only makes the compiler wait for ALL outstanding loads
@@ -88,13 +110,12 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
     // in flight through the compute phase): row (pq|..) of a single excitation, or the
     // diagonal row sum_{p in Ia} (pp|..) from dw.  Entries past n2 run into the next row or
     // the zero padding after the table and land in Wr slots no lane reads.
-    auto load_w = [&](T (&pre)[NFM], bool &neg, int ca) {
+    auto load_w = [&](T &pre, bool &neg, int ca) {
         const bool single = ca < nsxA;
         const u32 e = conn[single ? ca : 0];
-        const T *er = single ? eri + (i64)(sx_r(e) * n + sx_s(e)) * n2 + e0 : dw + Ia * AB_SRC_NWE + e0;
+        const T *er = single ? eri + (i64)(sx_r(e) * n + sx_s(e)) * n2 : dw + Ia * AB_SRC_NWE;
         neg = single && sx_neg(e);
-#pragma unroll
-        for (int f = 0; f < NFM; ++f) pre[f] = er[f * (BD / R)];
+        pre = er[fe];
     };
     // source elements of the round (chunk, connection ca); rounds are tracked by incremental
     // counters: a 64-bit division here would be expanded into vector code in the FMA phase.
@@ -108,14 +129,13 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
         for (int v = 0; v < NVS; ++v) xv[v] = xt[(Ja * NB + Jb) * nvst + v0 + v];
     };
 
-    T pre[NFM];
+    T pre;
     bool pneg = false;
     load_w(pre, pneg, 0);
     int wca = 0;  // connection whose W values are in pre
     if (DB == 2) {
         // round 0's row goes into buffer 0 now; pre then carries round 1's
-#pragma unroll
-        for (int f = 0; f < NFM; ++f) Wr[tid + f * BD] = pneg ? -pre[f] : pre[f];
+        store_w(Wr, pneg ? -pre : pre);
         wca = 1 % nca;
         load_w(pre, pneg, wca);
     }
@@ -181,16 +201,13 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
             __syncthreads();
             wr_off = (u32)((it & 1) * AB_SRC_NWE * R * sizeof(T));
             if (it + 1 < nit) {
-                T *Wn = Wr + ((it + 1) & 1) * AB_SRC_NWE * R;
-#pragma unroll
-                for (int f = 0; f < NFM; ++f) Wn[tid + f * BD] = pneg ? -pre[f] : pre[f];
+                store_w(Wr + ((it + 1) & 1) * AB_SRC_NWE * R, pneg ? -pre : pre);
             }
             wca = wca + 1 == nca ? 0 : wca + 1;
             load_w(pre, pneg, wca);
         } else {
             __syncthreads();  // everyone is done reading the previous W
-#pragma unroll
-            for (int f = 0; f < NFM; ++f) Wr[tid + f * BD] = pneg ? -pre[f] : pre[f];
+            store_w(Wr, pneg ? -pre : pre);
             __syncthreads();
             load_w(pre, pneg, ca + 1 == nca ? 0 : ca + 1);
         }
