@@ -31,6 +31,11 @@ constexpr int AB_SRC_NWE = 256;  // integral entries one W fill covers (n*n <= 2
 constexpr int AB_SRC_XPF = 3;    // alpha connections of source elements in flight
 constexpr int AB_SRC_BINOM = 160; // binomial table entries: n <= 16 orbitals x (KM + 2) <= 10
 constexpr int AB_SRC_CONN = 64;   // alpha connections per row: k (n - k) <= 64 for n <= 16
+// slot rows of LDS gathers issued ahead of the FMAs (measured at 16 orbitals: 2 rows is no
+// faster than 1, 3 is 2-4% slower)
+constexpr int AB_SRC_PD = 1;
+// keep slot addresses in VGPRs across rounds (NVS = 1, DB = 1 only): 1.5-3% faster there
+constexpr bool AB_SRC_HOIST = true;
 
 template <typename T, int NVS, int KM, int VM, int R, int SPLIT, int DB>
 __global__ __launch_bounds__(AB_SRC_NJ *SPLIT) __attribute__((amdgpu_waves_per_eu(2 * SPLIT, 2 * SPLIT)))
@@ -220,29 +225,37 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
             // a zero read from LDS after this round's barrier makes the row offsets opaque per
             // round, so the KH*VM slot addresses are formed in the loop (one add each) instead
             // of being hoisted into VGPRs (inline asm would do it too, but forces full waits)
-            const u32 lz = *reinterpret_cast<const u32 *>(smem_raw + zero_off);
+            // HOIST: the KH x VM slot addresses may stay in VGPRs across rounds (single-vector,
+            // single-buffer kernels, where the registers allow); otherwise a zero read from LDS
+            // after this round's barrier makes the row offsets opaque per round, so the
+            // addresses are formed in the loop (one add each) instead of being hoisted
+            constexpr bool hoist = AB_SRC_HOIST && NVS == 1 && DB == 1;
+            const u32 lz = hoist ? 0u : *reinterpret_cast<const u32 *>(smem_raw + zero_off);
             u32 ob[KH];
 #pragma unroll
             for (int i = 0; i < KH; ++i) ob[i] = oo[i] + wr_off + lz;
-            // software pipeline over the slot rows: row i+1's VM gathers are issued before row
-            // i's FMAs, so the LDS latency of one row hides behind the FMAs of the previous
-            // one; the scheduling barrier keeps the compiler from hoisting more rows of
-            // gathers (their registers would push the accumulators out)
-            T wv[2][VM];
+            // software pipeline over the slot rows: row i+PD's VM gathers are issued before
+            // row i's FMAs, so the LDS latency hides behind the FMAs of PD rows; the
+            // scheduling barrier keeps the compiler from hoisting more rows of gathers (their
+            // registers would push the accumulators out)
+            constexpr int PD = AB_SRC_PD < KH ? AB_SRC_PD : KH;
+            T wv[PD + 1][VM];
 #pragma unroll
-            for (int iv = 0; iv < VM; ++iv) wv[0][iv] = *reinterpret_cast<const T *>(smem_raw + ob[0] + vv[iv]);
+            for (int d = 0; d < PD; ++d)
+#pragma unroll
+                for (int iv = 0; iv < VM; ++iv) wv[d][iv] = *reinterpret_cast<const T *>(smem_raw + ob[d] + vv[iv]);
 #pragma unroll
             for (int i = 0; i < KH; ++i) {
-                if (i + 1 < KH) {
+                if (i + PD < KH) {
 #pragma unroll
                     for (int iv = 0; iv < VM; ++iv)
-                        wv[(i + 1) & 1][iv] = *reinterpret_cast<const T *>(smem_raw + ob[i + 1] + vv[iv]);
+                        wv[(i + PD) % (PD + 1)][iv] = *reinterpret_cast<const T *>(smem_raw + ob[i + PD] + vv[iv]);
                 }
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int iv = 0; iv < VM; ++iv)
 #pragma unroll
-                    for (int v = 0; v < NVS; ++v) acc[i][iv][v] += wv[i & 1][iv] * xq[0][v];
+                    for (int v = 0; v < NVS; ++v) acc[i][iv][v] += wv[i % (PD + 1)][iv] * xq[0][v];
             }
             T wd = 0;
 #pragma unroll
