@@ -6,7 +6,8 @@ ansatz order, davidson(), energy_objective) is not available, so the restatement
 oracle/glue.py and oracle/oracle.py drive them; the Pauli terms come from the reference's own
 generate_hamiltonian_real_64_128_f64.
 
-    python tests/golden/make_golden.py            # all fixtures (a few minutes on 8 cores)
+    python tests/golden/make_golden.py            # all fixtures (~15 minutes on 8 cores)
+    python tests/golden/make_golden.py config1    # one set: small | config0 | config1
 """
 from __future__ import annotations
 
@@ -178,7 +179,42 @@ def config0_fixtures(R):
     print("config0", meta, flush=True)
 
 
+def config1_fixtures(R):
+    """16 orbitals, 8+8 electrons (165,636,900 determinants): one reference sigma of the seeded
+    benchmark vector, kept as checksums and 4096 sampled entries, and the diagonal's head."""
+    n, na, nb = 16, 8, 8
+    h, eri = for_norb(n)
+    h1, g2 = one_body_layout(h), reference_layout(eri)
+    t0 = time.time()
+    S = ref_system(R, n, na, nb, h1, g2)
+    dim = S["dim"]
+    x = seeded(dim, 1000 + 16)
+    t1 = time.time()
+    print("config1 network", t1 - t0, "s", flush=True)
+    sigma = R.hvec(S["basis"], S["ham"], x)
+    t_sigma = time.time() - t1
+    print("config1 sigma", t_sigma, "s", flush=True)
+    del x
+    diag = R.diags(S["basis"], S["ham"], dim)
+    y = seeded(dim, 4242)
+    samp = np.sort(np.random.default_rng(99).choice(dim, 4096, replace=False))
+    np.savez_compressed(os.path.join(HERE, "config1_n16.npz"), sigma_sum=sigma.sum(),
+                        sigma_norm2=sigma @ sigma, sigma_dot_y=y @ sigma, sample_idx=samp,
+                        sample_val=sigma[samp], diag_head=diag[:256], diag_sum=diag.sum(),
+                        diag_sample=diag[samp], nterms=S["nterms"])
+    meta = dict(integrals_sha256=digest(h, eri), build_s=t1 - t0, sigma_s=t_sigma,
+                threads=os.environ.get("OMP_NUM_THREADS", "default"))
+    with open(os.path.join(HERE, "config1_timing.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    print("config1", meta, flush=True)
+
+
 if __name__ == "__main__":
     R = Reference()
-    small_fixtures(R)
-    config0_fixtures(R)
+    which = sys.argv[1:] or ["small", "config0", "config1"]
+    if "small" in which:
+        small_fixtures(R)
+    if "config0" in which:
+        config0_fixtures(R)
+    if "config1" in which:
+        config1_fixtures(R)

@@ -100,6 +100,8 @@ def test_tuned_sigma_matches_default_and_is_applied(torch_cuda, tmp_path, restor
     assert AT.get_config(AT.FP64) == entry["fp64"]["config"]
     assert AT.get_config(AT.FP32) == entry["fp32"]["config"]
     assert entry["davidson"]["block"] in (1, 2, 4, 8)
+    # every kernel family and launch variant reproduced the first candidate's sigma
+    assert not entry["fp64"]["rejected"] and not entry["fp32"]["rejected"]
     y = torch.empty_like(xs[0])
     F.hvec(xs[0], y)
     ys = [torch.empty_like(x) for x in xs]

@@ -20,7 +20,12 @@ def _dev(n, na, nb, seed=None):
     return B, H, otf.HipOTF_Functions(B, ham=H)
 
 
-@pytest.mark.parametrize("n,na,nb", SMALL + [(12, 6, 6)])
+# beyond 16 orbitals (36-bit interleaved determinants; the alpha-beta part falls back from the
+# source-lane kernel, which is sized for n <= 16)
+WIDE = [(17, 3, 3), (18, 2, 2), (18, 3, 2)]
+
+
+@pytest.mark.parametrize("n,na,nb", SMALL + [(12, 6, 6)] + WIDE)
 def test_hvec_matches_oracle(torch_cuda, n, na, nb):
     print(f"hvec n={n} na={na} nb={nb}", flush=True)
     S = oracle_system(n, na, nb)
@@ -35,7 +40,7 @@ def test_hvec_matches_oracle(torch_cuda, n, na, nb):
     assert np.max(np.abs(y - ref)) < 1e-12 * np.max(np.abs(ref))
 
 
-@pytest.mark.parametrize("n,na,nb", SMALL + [(12, 6, 6)])
+@pytest.mark.parametrize("n,na,nb", SMALL + [(12, 6, 6)] + WIDE)
 def test_diags_match_oracle(torch_cuda, n, na, nb):
     S = oracle_system(n, na, nb)
     ref = S.diags()
@@ -45,7 +50,7 @@ def test_diags_match_oracle(torch_cuda, n, na, nb):
     assert np.max(np.abs(host(d) - 0.25 - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
 
 
-@pytest.mark.parametrize("n,na,nb", [(6, 3, 3), (8, 3, 5), (12, 6, 6)])
+@pytest.mark.parametrize("n,na,nb", [(6, 3, 3), (8, 3, 5), (12, 6, 6), (18, 3, 2)])
 @pytest.mark.parametrize("k", [1, 2, 3, 4, 8, 9])
 def test_block_sigma(torch_cuda, n, na, nb, k):
     S = oracle_system(n, na, nb)
