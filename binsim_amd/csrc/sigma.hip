@@ -134,33 +134,45 @@ __global__ __launch_bounds__(256) void k_rowspmm(const T *__restrict__ X, T *__r
 // by the 4 waves.  Work items (tile-major: all row blocks of a column tile, then the next
 // tile) are dealt to the 8 XCDs in contiguous ranges, so each XCD sweeps whole column tiles.
 // ---------------------------------------------------------------------------------------
-template <typename T, int RB>
+// CPL consecutive columns per lane, up to 16 bytes (2 FP64 or 4 FP32 columns) when the row
+// length allows: the gathers are bound by the texture pipeline's per-lane work (TA ~85%, TD
+// ~92% busy at ~26 B/clk/CU with one FP64 column per lane), so wider lanes move more bytes per
+// cycle (FP64 sigma at 16 orbitals: 218 -> 209 ms)
+template <typename T>
+static int rs_cpl(i64 ncols) {
+    constexpr int wide = 16 / (int)sizeof(T);
+    if (ncols % wide == 0) return wide;
+    return ncols % 2 == 0 ? 2 : 1;
+}
+
+template <typename T, int RB, int CPL>
 __global__ __launch_bounds__(256) void k_rbspmm(const T *__restrict__ X, T *__restrict__ Y,
                                                 const int32_t *__restrict__ J,
                                                 const int32_t *__restrict__ P,
                                                 const T *__restrict__ C, i64 nrows, i64 ncols,
                                                 i64 nblk, i64 nwork, i64 per_xcd, int g0, int g1,
                                                 int accumulate) {
+    typedef T TV __attribute__((ext_vector_type(CPL)));
     const i64 b = blockIdx.x;
     const i64 w = (b & 7) * per_xcd + (b >> 3);
     if (w >= nwork) return;
     const i64 tile = w / nblk, blk = w - (w / nblk) * nblk;
-    const i64 col = tile * 256 + threadIdx.x;
+    const i64 col = (tile * 256 + threadIdx.x) * CPL;  // ncols is a multiple of CPL
     if (col >= ncols) return;
     const int eb = P[blk * (SS_RANGES + 1) + g0], ee = P[blk * (SS_RANGES + 1) + g1];
-    T acc[RB];
+    TV acc[RB];
 #pragma unroll
     for (int r = 0; r < RB; ++r) acc[r] = 0;
     const T *Xc = X + col;
+    auto gather = [&](int e) { return *reinterpret_cast<const TV *>(Xc + (i64)J[e] * ncols); };
     // batches of U gathers in flight per wave.  (Measured at 16 orbitals: issuing batch i + 1
-    // before batch i's FMAs, or buffer loads with a scalar row base, are 2-7% slower: the
-    // limit is the memory system under load, not the wave's own latency.)
+    // before batch i's FMAs, or buffer loads with a scalar row base, are 2-7% slower.)
     constexpr int U = 8;
     int e = eb;
     for (; e + U <= ee; e += U) {
-        T xv[U];
+        TV xv[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) xv[u] = Xc[(i64)J[e + u] * ncols];
+        for (int u = 0; u < U; ++u) xv[u] = gather(e + u);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const T *c = C + (i64)(e + u) * RB;
@@ -169,7 +181,7 @@ __global__ __launch_bounds__(256) void k_rbspmm(const T *__restrict__ X, T *__re
         }
     }
     for (; e < ee; ++e) {
-        const T x = Xc[(i64)J[e] * ncols];
+        const TV x = gather(e);
         const T *c = C + (i64)e * RB;
 #pragma unroll
         for (int r = 0; r < RB; ++r) acc[r] += c[r] * x;
@@ -178,7 +190,7 @@ __global__ __launch_bounds__(256) void k_rbspmm(const T *__restrict__ X, T *__re
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
         if (I0 + r < nrows) {
-            T *y = Y + (I0 + r) * ncols + col;
+            TV *y = reinterpret_cast<TV *>(Y + (I0 + r) * ncols + col);
             *y = accumulate ? *y + acc[r] : acc[r];
         }
     }
@@ -482,19 +494,32 @@ static size_t lds_budget() {
     return cached;
 }
 
-template <typename T, int RB>
-static void launch_rbspmm(const T *X, T *Y, const RowBlockTable &rt, i64 nrows, i64 ncols, int npass,
-                          hipStream_t st) {
-    const i64 ntiles = (ncols + 255) / 256;
+template <typename T, int RB, int CPL>
+static void launch_rbspmm_c(const T *X, T *Y, const RowBlockTable &rt, i64 nrows, i64 ncols, int npass,
+                            hipStream_t st) {
+    const i64 ntiles = (ncols + 256 * CPL - 1) / (256 * CPL);
     const i64 nwork = ntiles * rt.nblk;
     const i64 per_xcd = (nwork + 7) / 8;
     HIPOTF_REQUIRE(per_xcd * 8 < (1ll << 31), "row-block SpMM grid too large");
     const T *C = sizeof(T) == 4 ? (const T *)rt.c32.p : (const T *)rt.c64.p;
     for (int p = 0; p < npass; ++p)
-        hipLaunchKernelGGL((k_rbspmm<T, RB>), dim3((unsigned)(per_xcd * 8)), dim3(256), 0, st, X, Y,
+        hipLaunchKernelGGL((k_rbspmm<T, RB, CPL>), dim3((unsigned)(per_xcd * 8)), dim3(256), 0, st, X, Y,
                            rt.J.as<int32_t>(), rt.P.as<int32_t>(), C, nrows, ncols, rt.nblk, nwork,
                            per_xcd, p * SS_RANGES / npass, (p + 1) * SS_RANGES / npass, p > 0 ? 1 : 0);
     HIPOTF_CHECK(hipGetLastError());
+}
+
+template <typename T, int RB>
+static void launch_rbspmm(const T *X, T *Y, const RowBlockTable &rt, i64 nrows, i64 ncols, int npass,
+                          hipStream_t st) {
+    // (the buffers are 256-byte aligned and ncols is a multiple of CPL, so every lane's columns
+    // are aligned to their size)
+    const int cpl = rs_cpl<T>(ncols);
+    if constexpr (sizeof(T) == 4) {
+        if (cpl == 4) return launch_rbspmm_c<T, RB, 4>(X, Y, rt, nrows, ncols, npass, st);
+    }
+    if (cpl == 2) launch_rbspmm_c<T, RB, 2>(X, Y, rt, nrows, ncols, npass, st);
+    else launch_rbspmm_c<T, RB, 1>(X, Y, rt, nrows, ncols, npass, st);
 }
 
 // source-range passes of a row SpMM: the tuned value, or (0) the fewest passes (1, 2, 4, 8) for
@@ -516,7 +541,7 @@ static void launch_rowspmm(const T *X, T *Y, SameSpinTable &tab, const T *C, i64
     HIPOTF_REQUIRE(nrows == tab.N, "internal: same-spin table does not match the row count");
     const int rb = tuning(sizeof(T) == 4).rs_block;
     if (rb == 4 || rb == 8 || rb == 16) {
-        const int npass = rs_npass<T>(nrows, 256);
+        const int npass = rs_npass<T>(nrows, 256 * rs_cpl<T>(ncols));
         const RowBlockTable &rt = row_blocks(tab, rb);
         if (rb == 4) launch_rbspmm<T, 4>(X, Y, rt, nrows, ncols, npass, st);
         else if (rb == 8) launch_rbspmm<T, 8>(X, Y, rt, nrows, ncols, npass, st);
