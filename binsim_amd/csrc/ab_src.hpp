@@ -27,17 +27,17 @@
 #pragma once
 
 constexpr int AB_SRC_NJ = 512;   // source strings per chunk
-constexpr int AB_SRC_NWE = 256;  // integral entries one W fill covers (n*n <= 256)
+// integral entries one W fill covers (template NW): 256 for n*n <= 256, 512 for n*n <= 512
 constexpr int AB_SRC_XPF = 3;    // alpha connections of source elements in flight
-constexpr int AB_SRC_BINOM = 160; // binomial table entries: n <= 16 orbitals x (KM + 2) <= 10
-constexpr int AB_SRC_CONN = 64;   // alpha connections per row: k (n - k) <= 64 for n <= 16
+constexpr int AB_SRC_BINOM = 192; // binomial table entries: n <= 18 orbitals x (KM + 2) <= 10
+constexpr int AB_SRC_CONN = 88;   // alpha connections per row: k (n - k) <= 81 for n <= 18
 // slot rows of LDS gathers issued ahead of the FMAs (measured at 16 orbitals: 2 rows is no
 // faster than 1, 3 is 2-4% slower)
 constexpr int AB_SRC_PD = 1;
 // keep slot addresses in VGPRs across rounds (NVS = 1, DB = 1 only): 1.5-3% faster there
 constexpr bool AB_SRC_HOIST = true;
 
-template <typename T, int NVS, int KM, int VM, int R, int SPLIT, int DB>
+template <typename T, int NVS, int KM, int VM, int R, int SPLIT, int DB, int NW>
 __global__ __launch_bounds__(AB_SRC_NJ *SPLIT) __attribute__((amdgpu_waves_per_eu(2 * SPLIT, 2 * SPLIT)))
 void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, const u32 *__restrict__ sxA,
               const u32 *__restrict__ sxB, const u64 *__restrict__ strsA, const u64 *__restrict__ strsB,
@@ -51,20 +51,20 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
     // entry.  The block order is rotated with the entry so that the 64 lanes of one store
     // instruction spread evenly over the LDS banks.
     constexpr int NWV = BD / 64;                   // waves
-    constexpr int EPW = AB_SRC_NWE / NWV;          // entries per wave
+    constexpr int EPW = NW / NWV;                  // entries per wave
     constexpr int LPE = 64 / EPW;                  // lanes per entry
     constexpr int CB = 16 / (int)sizeof(T);        // copies per 16-byte block
     constexpr int NBLK = R / CB;                   // blocks per entry
     constexpr int BPL = NBLK / LPE;                // blocks stored per lane
-    static_assert(KM % SPLIT == 0 && NWV * EPW == AB_SRC_NWE && LPE * EPW == 64 && NBLK % LPE == 0 &&
+    static_assert(KM % SPLIT == 0 && NWV * EPW == NW && LPE * EPW == 64 && NBLK % LPE == 0 &&
                   BPL >= 1, "fill layout");
     typedef T TV __attribute__((ext_vector_type(CB)));
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n2 = n * n;
     // DB = 2: two replicated rows, the next connection's filled while the current one is read
     // (one barrier per connection instead of two)
-    T *Wr = reinterpret_cast<T *>(smem_raw);       // DB x AB_SRC_NWE * R replicated integral rows
-    T *Z = Wr + DB * AB_SRC_NWE * R;               // 16 zeros (reads of unused slots)
+    T *Wr = reinterpret_cast<T *>(smem_raw);       // DB x NW * R replicated integral rows
+    T *Z = Wr + DB * NW * R;                       // 16 zeros (reads of unused slots)
     T *S = Z + 16;                                 // NB * NVS destination row
     constexpr int BW = KM + 2;
     int *binom = reinterpret_cast<int *>(S + NB * NVS);  // C(p, j), p < n, j < BW
@@ -118,7 +118,7 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
     auto load_w = [&](T &pre, bool &neg, int ca) {
         const bool single = ca < nsxA;
         const u32 e = conn[single ? ca : 0];
-        const T *er = single ? eri + (i64)(sx_r(e) * n + sx_s(e)) * n2 : dw + Ia * AB_SRC_NWE;
+        const T *er = single ? eri + (i64)(sx_r(e) * n + sx_s(e)) * n2 : dw + Ia * NW;
         neg = single && sx_neg(e);
         pre = er[fe];
     };
@@ -166,7 +166,7 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
     bool live = false;
     // slots beyond (kB, vB) of the instantiated KM x VM grid read valid LDS words into
     // accumulators that are never scattered; their diagonal terms read the zeros Z
-    const u32 zero_off = (u32)(DB * AB_SRC_NWE * R * sizeof(T));
+    const u32 zero_off = (u32)(DB * NW * R * sizeof(T));
     i64 chunk = 0;
     int ca = 0;
     for (i64 it = 0; it < nit; ++it, advance(chunk, ca)) {
@@ -204,9 +204,9 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
             // this round's row (filled last round) is complete and the other buffer's last
             // readers are done: fill it with the next round's row, load the one after
             __syncthreads();
-            wr_off = (u32)((it & 1) * AB_SRC_NWE * R * sizeof(T));
+            wr_off = (u32)((it & 1) * NW * R * sizeof(T));
             if (it + 1 < nit) {
-                store_w(Wr + ((it + 1) & 1) * AB_SRC_NWE * R, pneg ? -pre : pre);
+                store_w(Wr + ((it + 1) & 1) * NW * R, pneg ? -pre : pre);
             }
             wca = wca + 1 == nca ? 0 : wca + 1;
             load_w(pre, pneg, wca);
@@ -229,7 +229,7 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
             // single-buffer kernels, where the registers allow); otherwise a zero read from LDS
             // after this round's barrier makes the row offsets opaque per round, so the
             // addresses are formed in the loop (one add each) instead of being hoisted
-            constexpr bool hoist = AB_SRC_HOIST && NVS == 1 && DB == 1;
+            constexpr bool hoist = AB_SRC_HOIST && NVS == 1 && DB == 1 && KM * VM <= 64;
             const u32 lz = hoist ? 0u : *reinterpret_cast<const u32 *>(smem_raw + zero_off);
             u32 ob[KH];
 #pragma unroll

@@ -610,14 +610,18 @@ SigmaTuning &tuning(int precision) {
 }
 static bool deterministic(int prec) { return tuning(prec).deterministic != 0; }
 
-template <typename T, int NVS, int R>
+template <typename T, int NVS, int R, int NW>
 static bool try_ab_src_r(DevHam *H, const T *xt, T *s1, int nvst, int v0, const T *eri, hipStream_t st);
 
-// dw[Ia][rt] = sum_{p in Ia} (pp|rt) for rt < n*n, 0 up to AB_SRC_NWE: the integral row of the
-// diagonal alpha connection, read by the source-lane kernel like the (pq|..) rows
+// entries per integral row of the source-lane kernel for n orbitals (n*n <= 512)
+static int ab_src_nw(int n) { return n * n <= 256 ? 256 : 512; }
+
+// dw[Ia][rt] = sum_{p in Ia} (pp|rt) for rt < n*n, 0 up to the row width NW: the integral row
+// of the diagonal alpha connection, read by the source-lane kernel like the (pq|..) rows
 template <typename T>
-__global__ __launch_bounds__(AB_SRC_NWE) void k_diag_w(const u64 *__restrict__ strsA,
-                                                       const double *__restrict__ eri, int n, T *__restrict__ dw) {
+__global__ __launch_bounds__(512) void k_diag_w(const u64 *__restrict__ strsA,
+                                                const double *__restrict__ eri, int n, int nw,
+                                                T *__restrict__ dw) {
     const i64 Ia = blockIdx.x;
     const int t = threadIdx.x, n2 = n * n;
     double v = 0.0;
@@ -629,7 +633,7 @@ __global__ __launch_bounds__(AB_SRC_NWE) void k_diag_w(const u64 *__restrict__ s
             v += eri[(i64)(p * n + p) * n2 + t];
         }
     }
-    dw[Ia * AB_SRC_NWE + t] = (T)v;
+    if (t < nw) dw[Ia * nw + t] = (T)v;
 }
 
 template <typename T>
@@ -637,10 +641,11 @@ static const T *diag_w_table(DevHam *H, hipStream_t st) {
     DevBuf &b = sizeof(T) == 8 ? H->dw64 : H->dw32;
     const DevSpin &A = *H->basis->pA;
     if (!b.p) {
+        const int nw = ab_src_nw(H->n);
         HIPOTF_REQUIRE(A.N > 0 && A.N < (1ll << 31), "alpha string count out of range");
-        b.alloc((size_t)A.N * AB_SRC_NWE * sizeof(T));
-        hipLaunchKernelGGL(k_diag_w<T>, dim3((unsigned)A.N), dim3(AB_SRC_NWE), 0, st, A.strs.as<u64>(),
-                           H->eri64.as<double>(), H->n, b.as<T>());
+        b.alloc((size_t)A.N * nw * sizeof(T));
+        hipLaunchKernelGGL(k_diag_w<T>, dim3((unsigned)A.N), dim3(nw), 0, st, A.strs.as<u64>(),
+                           H->eri64.as<double>(), H->n, nw, b.as<T>());
         HIPOTF_CHECK(hipGetLastError());
     }
     return b.as<T>();
@@ -654,11 +659,16 @@ static bool try_ab_src(DevHam *H, const T *xt, T *s1, int nvst, int v0, const T 
         // R copies of the integral row: 32 lanes of 4-byte reads are served together, so 32
         // copies make them conflict-free; 8-byte reads get 16 (LDS space).  (FP32 with 16
         // copies measured the same at 16 orbitals.)
-        return try_ab_src_r<T, NVS, sizeof(T) == 8 ? 16 : 32>(H, xt, s1, nvst, v0, eri, st);
+        if (H->n * H->n <= 256) return try_ab_src_r<T, NVS, sizeof(T) == 8 ? 16 : 32, 256>(H, xt, s1, nvst, v0, eri, st);
+        // 17-22 orbitals: 512-entry rows; only the single-vector FP32 kernel fits (the FP32
+        // destination row of 18 orbitals / 7 electrons is 127 KB of LDS, 16 copies of the row
+        // 32 KB; in FP64 the row alone exceeds the LDS and 77 accumulators the registers)
+        if constexpr (sizeof(T) == 4 && NVS == 1) return try_ab_src_r<T, NVS, 16, 512>(H, xt, s1, nvst, v0, eri, st);
+        return false;
     }
 }
 
-template <typename T, int NVS, int R>
+template <typename T, int NVS, int R, int NW>
 static bool try_ab_src_r(DevHam *H, const T *xt, T *s1, int nvst, int v0, const T *eri, hipStream_t st) {
     {
         DevBasis *B = H->basis;
@@ -668,16 +678,16 @@ static bool try_ab_src_r(DevHam *H, const T *xt, T *s1, int nvst, int v0, const 
         // threads per source string: 1 (512-thread blocks, 2 waves per SIMD, accumulators in
         // up to 256 VGPRs) or 2 (1024-thread blocks with half the slots per thread)
         constexpr int prec = sizeof(T) == 4;
-        const int split = tuning(prec).ab_split == 2 ? 2 : 1;
+        const int split = NW == 256 && tuning(prec).ab_split == 2 ? 2 : 1;
         const i64 NB = Bs.N;
         const int bd = AB_SRC_NJ * split;
-        if (n2 > AB_SRC_NWE || A.nsx > AB_SRC_CONN || n * (8 + 2) > AB_SRC_BINOM) return false;
-        // a W fill reads AB_SRC_NWE entries from the start of one (pq|..) row
-        static_assert(AB_SRC_NWE <= ERI_PAD + 1, "integral tables are padded for one fill past the last row");
-        // + the binomial table C(p, j), p < n <= 16, j < KM + 2 <= 10; two replicated rows
+        if (n2 > NW || ab_src_nw(n) != NW || A.nsx > AB_SRC_CONN || n * (8 + 2) > AB_SRC_BINOM) return false;
+        // a W fill reads NW entries from the start of one (pq|..) row
+        HIPOTF_REQUIRE(NW - n2 <= ERI_PAD, "internal: integral table padding too short for the row width");
+        // + the binomial table C(p, j), p < n <= 18, j < KM + 2 <= 10; two replicated rows
         // (one barrier per connection) if tuned so and they fit
         auto lds_for = [&](int db) {
-            return ((size_t)db * AB_SRC_NWE * R + 16 + (size_t)NB * NVS) * sizeof(T) +
+            return ((size_t)db * NW * R + 16 + (size_t)NB * NVS) * sizeof(T) +
                    (AB_SRC_BINOM + AB_SRC_CONN) * 4;
         };
         const size_t lds_max = std::min<size_t>(lds_budget(), 160 * 1024);
@@ -696,19 +706,28 @@ static bool try_ab_src_r(DevHam *H, const T *xt, T *s1, int nvst, int v0, const 
                                dw, n, A.N, NB, A.nsx, kB, vB, deterministic(prec) ? 1 : 0);
             HIPOTF_CHECK(hipGetLastError());
         };
-        auto pick = [&](auto split_c, auto db_c) {
-            constexpr int SPLIT = decltype(split_c)::value, DB = decltype(db_c)::value;
-            if (kB <= 4 && vB <= 4) go(k_ab_src<T, NVS, 4, 4, R, SPLIT, DB>);
-            else if (kB <= 6 && vB <= 6) go(k_ab_src<T, NVS, 6, 6, R, SPLIT, DB>);
-            else if constexpr (sizeof(T) == 8 && NVS == 2) return false;  // 128 FP64 accumulators: spills
-            else if (kB <= 8 && vB <= 8) go(k_ab_src<T, NVS, 8, 8, R, SPLIT, DB>);
-            else return false;
-            return true;
-        };
-        using I1 = std::integral_constant<int, 1>;
-        using I2 = std::integral_constant<int, 2>;
-        if (split == 2) return db == 2 ? pick(I2{}, I2{}) : pick(I2{}, I1{});
-        return db == 2 ? pick(I1{}, I2{}) : pick(I1{}, I1{});
+        if constexpr (NW == 512) {
+            // 17-18 orbitals: up to 8 occupied and 12 virtual orbitals per beta string
+            if (kB <= 8 && vB <= 12 && db == 1) {
+                go(k_ab_src<T, NVS, 8, 12, R, 1, 1, NW>);
+                return true;
+            }
+            return false;
+        } else {
+            auto pick = [&](auto split_c, auto db_c) {
+                constexpr int SPLIT = decltype(split_c)::value, DB = decltype(db_c)::value;
+                if (kB <= 4 && vB <= 4) go(k_ab_src<T, NVS, 4, 4, R, SPLIT, DB, NW>);
+                else if (kB <= 6 && vB <= 6) go(k_ab_src<T, NVS, 6, 6, R, SPLIT, DB, NW>);
+                else if constexpr (sizeof(T) == 8 && NVS == 2) return false;  // 128 FP64 accumulators: spills
+                else if (kB <= 8 && vB <= 8) go(k_ab_src<T, NVS, 8, 8, R, SPLIT, DB, NW>);
+                else return false;
+                return true;
+            };
+            using I1 = std::integral_constant<int, 1>;
+            using I2 = std::integral_constant<int, 2>;
+            if (split == 2) return db == 2 ? pick(I2{}, I2{}) : pick(I2{}, I1{});
+            return db == 2 ? pick(I1{}, I2{}) : pick(I1{}, I1{});
+        }
     }
 }
 

@@ -20,9 +20,10 @@ def _dev(n, na, nb, seed=None):
     return B, H, otf.HipOTF_Functions(B, ham=H)
 
 
-# beyond 16 orbitals (36-bit interleaved determinants; the alpha-beta part falls back from the
-# source-lane kernel, which is sized for n <= 16)
-WIDE = [(17, 3, 3), (18, 2, 2), (18, 3, 2)]
+# beyond 16 orbitals (36-bit interleaved determinants).  The alpha-beta part runs on the
+# fallback kernels, except in FP32 with at most 12 beta holes ((18, 1, 6), (17, 2, 5)), where the
+# source-lane kernel's 512-entry-row form runs
+WIDE = [(17, 3, 3), (18, 2, 2), (18, 3, 2), (18, 1, 6), (17, 2, 5)]
 
 
 @pytest.mark.parametrize("n,na,nb", SMALL + [(12, 6, 6)] + WIDE)
@@ -50,7 +51,7 @@ def test_diags_match_oracle(torch_cuda, n, na, nb):
     assert np.max(np.abs(host(d) - 0.25 - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
 
 
-@pytest.mark.parametrize("n,na,nb", [(6, 3, 3), (8, 3, 5), (12, 6, 6), (18, 3, 2)])
+@pytest.mark.parametrize("n,na,nb", [(6, 3, 3), (8, 3, 5), (12, 6, 6), (18, 3, 2), (18, 1, 6)])
 @pytest.mark.parametrize("k", [1, 2, 3, 4, 8, 9])
 def test_block_sigma(torch_cuda, n, na, nb, k):
     S = oracle_system(n, na, nb)
