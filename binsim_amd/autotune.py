@@ -35,17 +35,19 @@ import numpy as np
 
 from . import lib as L
 
-FIELDS = ("ab_kernel", "ab_split", "ab_db", "ab_nvs", "rs_passes", "deterministic", "rs_block")
+FIELDS = ("ab_kernel", "ab_split", "ab_db", "ab_nvs", "rs_passes", "deterministic", "rs_block",
+          "sweep_mode")
 FP64, FP32 = 0, 1
 DEFAULT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 
 _RANGES = {"ab_kernel": (0, 1, 2, 3), "ab_split": (1, 2), "ab_db": (1, 2), "ab_nvs": (1, 2),
-           "rs_passes": (0, 1, 2, 4, 8), "deterministic": (0, 1), "rs_block": (0, 4, 8, 16)}
+           "rs_passes": (0, 1, 2, 4, 8), "deterministic": (0, 1), "rs_block": (0, 4, 8, 16),
+           "sweep_mode": (0, 1, 2)}
 
 
 def validate(cfg: dict) -> dict:
     """Return ``cfg`` completed with defaults; raise ValueError on an unknown or bad field."""
-    out = dict(zip(FIELDS, (0, 1, 1, 2, 0, 0, 8)))
+    out = dict(zip(FIELDS, (0, 1, 1, 2, 0, 0, 8, 0)))
     for k, v in cfg.items():
         if k not in _RANGES:
             raise ValueError(f"unknown tuning field {k!r}")

@@ -415,12 +415,12 @@ int hipotf_prepare(void *basis, void *net) {
 }
 
 // launch choices of the sigma kernels (process-wide, per precision 0 FP64 / 1 FP32):
-// v[0..n) = {ab_kernel, ab_split, ab_db, ab_nvs, rs_passes, deterministic, rs_block}; values
-// out of range are rejected
+// v[0..n) = {ab_kernel, ab_split, ab_db, ab_nvs, rs_passes, deterministic, rs_block,
+// sweep_mode}; values out of range are rejected
 int hipotf_set_tuning(int precision, const int32_t *v, int n) {
     API_TRY
     HIPOTF_REQUIRE(precision == 0 || precision == 1, "tuning: precision must be 0 or 1");
-    HIPOTF_REQUIRE(v != nullptr && n == TUNING_FIELDS, "tuning: expected 7 values");
+    HIPOTF_REQUIRE(v != nullptr && n == TUNING_FIELDS, "tuning: expected 8 values");
     HIPOTF_REQUIRE(v[0] >= 0 && v[0] <= 3, "tuning: ab_kernel must be 0..3");
     HIPOTF_REQUIRE(v[1] == 1 || v[1] == 2, "tuning: ab_split must be 1 or 2");
     HIPOTF_REQUIRE(v[2] == 1 || v[2] == 2, "tuning: ab_db must be 1 or 2");
@@ -429,6 +429,7 @@ int hipotf_set_tuning(int precision, const int32_t *v, int n) {
                    "tuning: rs_passes must be 0, 1, 2, 4 or 8");
     HIPOTF_REQUIRE(v[5] == 0 || v[5] == 1, "tuning: deterministic must be 0 or 1");
     HIPOTF_REQUIRE(v[6] == 0 || v[6] == 4 || v[6] == 8 || v[6] == 16, "tuning: rs_block must be 0, 4, 8 or 16");
+    HIPOTF_REQUIRE(v[7] >= 0 && v[7] <= 2, "tuning: sweep_mode must be 0, 1 or 2");
     SigmaTuning &t = tuning(precision);
     t.ab_kernel = v[0];
     t.ab_split = v[1];
@@ -437,6 +438,7 @@ int hipotf_set_tuning(int precision, const int32_t *v, int n) {
     t.rs_passes = v[4];
     t.deterministic = v[5];
     t.rs_block = v[6];
+    t.sweep_mode = v[7];
     return 0;
     API_CATCH(1)
 }
@@ -444,7 +446,7 @@ int hipotf_set_tuning(int precision, const int32_t *v, int n) {
 int hipotf_get_tuning(int precision, int32_t *v, int n) {
     API_TRY
     HIPOTF_REQUIRE(precision == 0 || precision == 1, "tuning: precision must be 0 or 1");
-    HIPOTF_REQUIRE(v != nullptr && n == TUNING_FIELDS, "tuning: expected 7 values");
+    HIPOTF_REQUIRE(v != nullptr && n == TUNING_FIELDS, "tuning: expected 8 values");
     const SigmaTuning &t = tuning(precision);
     v[0] = t.ab_kernel;
     v[1] = t.ab_split;
@@ -453,6 +455,7 @@ int hipotf_get_tuning(int precision, int32_t *v, int n) {
     v[4] = t.rs_passes;
     v[5] = t.deterministic;
     v[6] = t.rs_block;
+    v[7] = t.sweep_mode;
     return 0;
     API_CATCH(1)
 }

@@ -350,27 +350,32 @@ struct SegGen {
 
 enum { SEG_ROW_A = 0, SEG_ROW_B = 1, SEG_PAIR = 2, SEG_SINGLE = 3 };
 
-// one block = one row of nel elements; BACK: adjoint step on (l, r) with gradient partials
-template <bool BACK>
-__global__ __launch_bounds__(256) void k_seg_row(double *__restrict__ l, double *__restrict__ r,
-                                                 i64 nrows, i64 nel, const u64 *__restrict__ row_strs,
-                                                 const u64 *__restrict__ el_strs, int row_is_alpha,
-                                                 const SegGen *__restrict__ gens, int ng,
-                                                 double *__restrict__ partial) {
+// one block = one row of nel elements; BACK: adjoint step on (l, r) with gradient partials.
+// GL: the rows stay in global memory (they do not fit the LDS): the block updates them in place,
+// one generator after the other, and the launch is sized to one block per CU so that the rows
+// in flight stay in the L2 / Infinity Cache across the segment's generators
+template <bool BACK, bool GL>
+__global__ __launch_bounds__(GL ? 1024 : 256) void k_seg_row(double *__restrict__ l, double *__restrict__ r,
+                                                             i64 nrows, i64 nel, const u64 *__restrict__ row_strs,
+                                                             const u64 *__restrict__ el_strs, int row_is_alpha,
+                                                             const SegGen *__restrict__ gens, int ng,
+                                                             double *__restrict__ partial) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    double *xl = reinterpret_cast<double *>(smem_raw);
-    double *xr = xl + nel;
     const i64 row = blockIdx.x;
     if (row >= nrows) return;
     const int tid = threadIdx.x, bd = blockDim.x;
     const u64 rs = row_strs[row];
     double *gl = l + row * nel;
     double *gr = BACK ? r + row * nel : nullptr;
-    for (i64 t = tid; t < nel; t += bd) {
-        xl[t] = gl[t];
-        if (BACK) xr[t] = gr[t];
+    double *xl = GL ? gl : reinterpret_cast<double *>(smem_raw);
+    double *xr = GL ? gr : xl + nel;
+    if (!GL) {
+        for (i64 t = tid; t < nel; t += bd) {
+            xl[t] = gl[t];
+            if (BACK) xr[t] = gr[t];
+        }
+        __syncthreads();
     }
-    __syncthreads();
     for (int kk = 0; kk < ng; ++kk) {
         const int k = BACK ? ng - 1 - kk : kk;
         const SegGen sg = gens[k];
@@ -399,41 +404,53 @@ __global__ __launch_bounds__(256) void k_seg_row(double *__restrict__ l, double 
             const double t = block_sum(acc);
             if (tid == 0) partial[(i64)k * nrows + row] = t;
         }
-        __syncthreads();
+        __syncthreads();  // (GL: orders this generator's global updates before the next one's reads)
     }
-    for (i64 t = tid; t < nel; t += bd) {
-        gl[t] = xl[t];
-        if (BACK) gr[t] = xr[t];
+    if (!GL) {
+        for (i64 t = tid; t < nel; t += bd) {
+            gl[t] = xl[t];
+            if (BACK) gr[t] = xr[t];
+        }
     }
 }
 
 // one block = one alpha pair (u_a, w_a) of the segment's alpha pattern; rows of NB elements
-template <bool BACK>
-__global__ __launch_bounds__(256) void k_seg_pair(double *__restrict__ l, double *__restrict__ r,
-                                                  const int32_t *__restrict__ alist, i64 npairs, i64 NB,
-                                                  const u64 *__restrict__ strsA,
-                                                  const u64 *__restrict__ strsB,
-                                                  const SegGen *__restrict__ gens, int ng,
-                                                  double *__restrict__ partial) {
+// (the u and w strings of one pattern are disjoint, so blocks own disjoint row pairs).  GL: in
+// place in global memory, as in k_seg_row
+template <bool BACK, bool GL>
+__global__ __launch_bounds__(GL ? 1024 : 256) void k_seg_pair(double *__restrict__ l, double *__restrict__ r,
+                                                              const int32_t *__restrict__ alist, i64 npairs, i64 NB,
+                                                              const u64 *__restrict__ strsA,
+                                                              const u64 *__restrict__ strsB,
+                                                              const SegGen *__restrict__ gens, int ng,
+                                                              double *__restrict__ partial) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    double *l0 = reinterpret_cast<double *>(smem_raw);
-    double *l1 = l0 + NB;
-    double *r0 = l1 + NB;
-    double *r1 = r0 + NB;
     const i64 p = blockIdx.x;
     if (p >= npairs) return;
     const int tid = threadIdx.x, bd = blockDim.x;
     const int32_t ua = alist[2 * p], wa = alist[2 * p + 1];
     const u64 as = strsA[ua];
-    for (i64 t = tid; t < NB; t += bd) {
-        l0[t] = l[(i64)ua * NB + t];
-        l1[t] = l[(i64)wa * NB + t];
-        if (BACK) {
-            r0[t] = r[(i64)ua * NB + t];
-            r1[t] = r[(i64)wa * NB + t];
+    double *l0, *l1, *r0, *r1;
+    if (GL) {
+        l0 = l + (i64)ua * NB;
+        l1 = l + (i64)wa * NB;
+        r0 = BACK ? r + (i64)ua * NB : nullptr;
+        r1 = BACK ? r + (i64)wa * NB : nullptr;
+    } else {
+        l0 = reinterpret_cast<double *>(smem_raw);
+        l1 = l0 + NB;
+        r0 = l1 + NB;
+        r1 = r0 + NB;
+        for (i64 t = tid; t < NB; t += bd) {
+            l0[t] = l[(i64)ua * NB + t];
+            l1[t] = l[(i64)wa * NB + t];
+            if (BACK) {
+                r0[t] = r[(i64)ua * NB + t];
+                r1[t] = r[(i64)wa * NB + t];
+            }
         }
+        __syncthreads();
     }
-    __syncthreads();
     for (int kk = 0; kk < ng; ++kk) {
         const int k = BACK ? ng - 1 - kk : kk;
         const SegGen sg = gens[k];
@@ -461,14 +478,16 @@ __global__ __launch_bounds__(256) void k_seg_pair(double *__restrict__ l, double
             const double t = block_sum(acc);
             if (tid == 0) partial[(i64)k * npairs + p] = t;
         }
-        __syncthreads();
+        __syncthreads();  // (GL: orders this generator's global updates before the next one's reads)
     }
-    for (i64 t = tid; t < NB; t += bd) {
-        l[(i64)ua * NB + t] = l0[t];
-        l[(i64)wa * NB + t] = l1[t];
-        if (BACK) {
-            r[(i64)ua * NB + t] = r0[t];
-            r[(i64)wa * NB + t] = r1[t];
+    if (!GL) {
+        for (i64 t = tid; t < NB; t += bd) {
+            l[(i64)ua * NB + t] = l0[t];
+            l[(i64)wa * NB + t] = l1[t];
+            if (BACK) {
+                r[(i64)ua * NB + t] = r0[t];
+                r[(i64)wa * NB + t] = r1[t];
+            }
         }
     }
 }
@@ -502,6 +521,7 @@ __global__ __launch_bounds__(256) void k_transpose(const double *__restrict__ in
 
 struct Segment {
     int kind, first, count, pa;
+    bool glob;  // rows do not fit the LDS: the global-memory form of the segment kernel
 };
 
 static size_t seg_lds_budget() {
@@ -517,23 +537,30 @@ static std::vector<Segment> plan_sweep(DevPool *P, const i64 *idxs, i64 n, int n
     const DevBasis *B = P->basis;
     const i64 NA = B->pA->N, NB = B->pB->N;
     const size_t budget = seg_lds_budget();
-    const bool rowA_ok = (size_t)NB * nvec * 8 <= budget;
-    const bool rowB_ok = (size_t)NA * nvec * 8 <= budget;
-    const bool pair_ok = (size_t)NB * 2 * 2 * 8 <= budget;  // kernel reserves 4 rows
+    // rows that do not fit the LDS are updated in place in global memory (segment.glob);
+    // sweep_mode 1 takes that form for every segment, 2 launches per generator
+    const int mode = tuning(0).sweep_mode;
+    const bool rowA_ok = mode == 0 && (size_t)NB * nvec * 8 <= budget;
+    const bool rowB_ok = mode == 0 && (size_t)NA * nvec * 8 <= budget;
+    const bool pair_ok = mode == 0 && (size_t)NB * 2 * 2 * 8 <= budget;  // kernel reserves 4 rows
     std::vector<Segment> segs;
     for (i64 i = 0; i < n;) {
         const Generator &G = P->gens[idxs[i]];
-        Segment s{SEG_SINGLE, (int)i, 1, G.pa};
+        Segment s{SEG_SINGLE, (int)i, 1, G.pa, false};
         auto gen = [&](i64 j) -> const Generator & { return P->gens[idxs[j]]; };
-        if (G.kind == 2 && rowA_ok) {
+        if (mode == 2) {
+            // one launch per generator
+        } else if (G.kind == 2) {
             s.kind = SEG_ROW_A;
+            s.glob = !rowA_ok;
             while (i + s.count < n && gen(i + s.count).kind == 2) ++s.count;
-        } else if ((G.kind == 3 || (G.kind == 1 && i + 1 < n && gen(i + 1).kind == 3 && gen(i + 1).pa == G.pa)) &&
-                   pair_ok) {
+        } else if (G.kind == 3 || (G.kind == 1 && i + 1 < n && gen(i + 1).kind == 3 && gen(i + 1).pa == G.pa)) {
             s.kind = SEG_PAIR;
+            s.glob = !pair_ok;
             while (i + s.count < n && gen(i + s.count).pa == G.pa && gen(i + s.count).kind != 2) ++s.count;
-        } else if (G.kind == 1 && rowB_ok) {
+        } else if (G.kind == 1) {
             s.kind = SEG_ROW_B;
+            s.glob = !rowB_ok;
             while (i + s.count < n && gen(i + s.count).kind == 1 &&
                    !(i + s.count + 1 < n && gen(i + s.count + 1).kind == 3 &&
                      gen(i + s.count + 1).pa == gen(i + s.count).pa))
@@ -580,6 +607,9 @@ void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, doub
         if (n) HIPOTF_CHECK(hipMemcpyAsync(P->segdesc.p, host.data(), n * sizeof(SegGen), hipMemcpyHostToDevice, st));
         const SegGen *dg = P->segdesc.as<SegGen>();
         const int nvec = back ? 2 : 1;
+        // global-memory segments: LDS reserved only to hold one block per CU, so the rows the
+        // blocks in flight work on (256 CUs x up to 4 rows) stay within the L2 / Infinity Cache
+        const size_t glob_smem = seg_lds_budget() / 2 + 4096;
         for (size_t si = 0; si < segs.size(); ++si) {
             const Segment &s = back ? segs[segs.size() - 1 - si] : segs[si];
             if (s.kind == SEG_SINGLE) {
@@ -604,7 +634,8 @@ void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, doub
                     l = P->tl.as<double>();
                     r = P->tr.as<double>();
                 }
-                const size_t smem = (size_t)nel * nvec * sizeof(double);
+                const size_t smem = s.glob ? glob_smem : (size_t)nel * nvec * sizeof(double);
+                const int bd = s.glob ? 1024 : 256;
                 double *part = nullptr;
                 if (back) {
                     P->segpart.ensure((size_t)s.count * nrows * sizeof(double));
@@ -612,17 +643,18 @@ void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, doub
                 }
                 const u64 *rstr = A ? B->pA->strs.as<u64>() : B->pB->strs.as<u64>();
                 const u64 *estr = A ? B->pB->strs.as<u64>() : B->pA->strs.as<u64>();
+                auto go_row = [&](auto kern, double *rr, double *pp) {
+                    HIPOTF_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                     (int)smem));
+                    hipLaunchKernelGGL(kern, dim3((unsigned)nrows), dim3(bd), smem, st, l, rr, nrows, nel, rstr,
+                                       estr, A ? 1 : 0, dg + s.first, s.count, pp);
+                };
                 if (back) {
-                    HIPOTF_CHECK(hipFuncSetAttribute((const void *)k_seg_row<true>,
-                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-                    hipLaunchKernelGGL(k_seg_row<true>, dim3((unsigned)nrows), dim3(256), smem, st, l, r, nrows,
-                                       nel, rstr, estr, A ? 1 : 0, dg + s.first, s.count, part);
+                    if (s.glob) go_row(k_seg_row<true, true>, r, part);
+                    else go_row(k_seg_row<true, false>, r, part);
                 } else {
-                    HIPOTF_CHECK(hipFuncSetAttribute((const void *)k_seg_row<false>,
-                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-                    hipLaunchKernelGGL(k_seg_row<false>, dim3((unsigned)nrows), dim3(256), smem, st, l,
-                                       (double *)nullptr, nrows, nel, rstr, estr, A ? 1 : 0, dg + s.first,
-                                       s.count, (double *)nullptr);
+                    if (s.glob) go_row(k_seg_row<false, true>, (double *)nullptr, (double *)nullptr);
+                    else go_row(k_seg_row<false, false>, (double *)nullptr, (double *)nullptr);
                 }
                 HIPOTF_CHECK(hipGetLastError());
                 if (!A) {
@@ -640,22 +672,26 @@ void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, doub
                     if (back) HIPOTF_CHECK(hipMemsetAsync(dres + s.first, 0, s.count * sizeof(double), st));
                     continue;
                 }
-                const size_t smem = (size_t)NB * 4 * sizeof(double);
+                const size_t smem = s.glob ? glob_smem : (size_t)NB * 4 * sizeof(double);
+                const int bd = s.glob ? 1024 : 256;
                 double *part = nullptr;
                 if (back) {
                     P->segpart.ensure((size_t)s.count * np * sizeof(double));
                     part = P->segpart.as<double>();
-                    HIPOTF_CHECK(hipFuncSetAttribute((const void *)k_seg_pair<true>,
-                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-                    hipLaunchKernelGGL(k_seg_pair<true>, dim3((unsigned)np), dim3(256), smem, st, psi, lam,
-                                       LA.uw.as<int32_t>(), np, NB, B->pA->strs.as<u64>(),
-                                       B->pB->strs.as<u64>(), dg + s.first, s.count, part);
+                }
+                auto go_pair = [&](auto kern, double *rr, double *pp) {
+                    HIPOTF_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                     (int)smem));
+                    hipLaunchKernelGGL(kern, dim3((unsigned)np), dim3(bd), smem, st, psi, rr, LA.uw.as<int32_t>(),
+                                       np, NB, B->pA->strs.as<u64>(), B->pB->strs.as<u64>(), dg + s.first,
+                                       s.count, pp);
+                };
+                if (back) {
+                    if (s.glob) go_pair(k_seg_pair<true, true>, lam, part);
+                    else go_pair(k_seg_pair<true, false>, lam, part);
                 } else {
-                    HIPOTF_CHECK(hipFuncSetAttribute((const void *)k_seg_pair<false>,
-                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-                    hipLaunchKernelGGL(k_seg_pair<false>, dim3((unsigned)np), dim3(256), smem, st, psi,
-                                       (double *)nullptr, LA.uw.as<int32_t>(), np, NB, B->pA->strs.as<u64>(),
-                                       B->pB->strs.as<u64>(), dg + s.first, s.count, (double *)nullptr);
+                    if (s.glob) go_pair(k_seg_pair<false, true>, (double *)nullptr, (double *)nullptr);
+                    else go_pair(k_seg_pair<false, false>, (double *)nullptr, (double *)nullptr);
                 }
                 HIPOTF_CHECK(hipGetLastError());
                 if (back)

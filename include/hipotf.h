@@ -96,11 +96,13 @@ int64_t hipotf_num_generators(void *net);
 int hipotf_prepare(void *basis, void *net);
 /* launch choices of the sigma kernels, process-wide (no reference counterpart; the
  * reference's tiles are compile-time, src/include/ham/otf.hpp:6-9 and cuda/common.cuh:7-12):
- * v[7] = {alpha-beta kernel 0 auto | 1 source-lane | 2 register | 3 chunked, threads per
+ * v[8] = {alpha-beta kernel 0 auto | 1 source-lane | 2 register | 3 chunked, threads per
  * source string 1|2, integral-row buffers 1|2, FP32 vectors per alpha-beta launch 1|2,
  * row-SpMM source-range passes 1|2|4|8 (0: as many as keep a column tile's gathered slice
  * within an XCD's L2), deterministic 0|1, same-spin destination rows per
- * gathered source row 4|8|16 (0: one row per wave)}, for the FP64 (precision 0) or the
+ * gathered source row 4|8|16 (0: one row per wave), VQE rotation sweeps 0 fused (rows in LDS
+ * where they fit, else in global memory) | 1 fused with rows in global memory | 2 one launch
+ * per generator (FP64 table only)}, for the FP64 (precision 0) or the
  * FP32 (precision 1) kernels.  binsim_amd/autotune.py measures the candidates and persists its
  * choice as JSON.  Return 0 on success. */
 int hipotf_set_tuning(int precision, const int32_t *v, int n);

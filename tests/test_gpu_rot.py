@@ -75,10 +75,15 @@ def test_fused_vqe_energy_grad(torch_cuda, n, na, nb):
     assert np.max(np.abs(host(psi) - lv)) < 1e-12  # reverse sweep uncomputes the state
 
 
-def test_vqe_config2_matches_reference_fixture(torch_cuda):
+@pytest.mark.parametrize("sweep_mode", [0, 1, 2])
+def test_vqe_config2_matches_reference_fixture(torch_cuda, sweep_mode):
     """12 orbitals, 6+6 electrons, all 1,818 UCC parameters, theta ~ U[-0.05, 0.05]: the fused
-    energy + adjoint gradient equal the values the compiled reference produced (golden)."""
+    energy + adjoint gradient equal the values the compiled reference produced (golden), for
+    each sweep form: fused segments with rows in LDS (0; everything fits at this size), fused
+    segments updating the rows in global memory (1; the form larger systems take), and one
+    launch per generator (2)."""
     import os
+    from binsim_amd import autotune as AT
     f = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config0_n12.npz"))
     B, H, P, F, sos = _dev(12, 6, 6)
     th = f["vqe_thetas"]
@@ -86,7 +91,12 @@ def test_vqe_config2_matches_reference_fixture(torch_cuda):
     psi = torch_cuda.zeros(B.dim, dtype=torch_cuda.float64, device="cuda")
     psi[0] = 1.0
     lam = torch_cuda.zeros_like(psi)
-    e, g, d2, nrm = F.vqe_energy_grad(np.arange(len(th)), th, psi, lam)
+    saved = AT.get_config(AT.FP64)
+    try:
+        AT.set_config(AT.FP64, dict(saved, sweep_mode=sweep_mode))
+        e, g, d2, nrm = F.vqe_energy_grad(np.arange(len(th)), th, psi, lam)
+    finally:
+        AT.set_config(AT.FP64, saved)
     assert abs(e - float(f["vqe_energy"])) < 1e-11
     assert abs(d2 - float(f["vqe_d2h"])) < 1e-9
     assert np.max(np.abs(g - f["vqe_grad"])) < 1e-11
