@@ -187,6 +187,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pmc-unit", action="store_true",
                     help="one FP64 sigma and one plain copy of the same size (counter runs)")
+    ap.add_argument("--vqe-only", type=int, choices=(12, 16), default=None,
+                    help="only the VQE energy + adjoint gradient at 12 or 16 orbitals (profiling runs)")
     ap.add_argument("--tune", action="store_true",
                     help="run the sigma autotuner on the workload first (about a minute) instead "
                          "of applying the stored choice")
@@ -197,6 +199,11 @@ def main():
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), "no GPU visible"
+    if args.vqe_only:
+        m = args.vqe_only
+        r = vqe(torch, m, m // 2, m // 2, nfd=0, seed_off=2000, amp=0.05 if m == 12 else 0.02)
+        print(json.dumps({"vqe": m, **r}))
+        return
     n, na, nb = 16, 8, 8
     t = time.perf_counter()
     B, H, _, F = build(n, na, nb)
