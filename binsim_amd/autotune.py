@@ -42,7 +42,7 @@ DEFAULT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.
 
 _RANGES = {"ab_kernel": (0, 1, 2, 3), "ab_split": (1, 2), "ab_db": (1, 2), "ab_nvs": (1, 2),
            "rs_passes": (0, 1, 2, 4, 8), "deterministic": (0, 1), "rs_block": (0, 4, 8, 16),
-           "sweep_mode": (0, 1, 2)}
+           "sweep_mode": (0, 1, 2, 3)}
 
 
 def validate(cfg: dict) -> dict:

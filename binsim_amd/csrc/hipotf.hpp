@@ -204,9 +204,10 @@ struct SigmaTuning {
     int deterministic = 0;  // 1: alpha-beta scatter wave by wave (bitwise reproducible)
     int rs_block = 8;       // row SpMM: destination rows sharing each gathered source row
                             // (4, 8, 16), or 0 for one row per wave
-    int sweep_mode = 0;     // VQE rotation sweeps (FP64 table): 0 fused segments with rows in
-                            // LDS where they fit, else in global memory; 1 fused, rows always
-                            // in global memory; 2 one launch per generator
+    int sweep_mode = 0;     // VQE rotation sweeps (FP64 table): 0 fused segments with as many
+                            // rows in LDS as fit, the others in global memory; 1 fused, rows
+                            // always in global memory; 2 one launch per generator; 3 fused,
+                            // at most one row per block in LDS
 };
 SigmaTuning &tuning(int precision);  // 0: FP64 kernels, 1: FP32 kernels
 constexpr int TUNING_FIELDS = 8;

@@ -351,11 +351,12 @@ struct SegGen {
 enum { SEG_ROW_A = 0, SEG_ROW_B = 1, SEG_PAIR = 2, SEG_SINGLE = 3 };
 
 // one block = one row of nel elements; BACK: adjoint step on (l, r) with gradient partials.
-// GL: the rows stay in global memory (they do not fit the LDS): the block updates them in place,
-// one generator after the other, and the launch is sized to one block per CU so that the rows
-// in flight stay in the L2 / Infinity Cache across the segment's generators
-template <bool BACK, bool GL>
-__global__ __launch_bounds__(GL ? 1024 : 256) void k_seg_row(double *__restrict__ l, double *__restrict__ r,
+// NST of the block's rows (l, then r) are staged in LDS; rows that do not fit stay in global
+// memory and are updated in place, one generator after the other, with the launch sized to
+// one block per CU so that the rows in flight stay in the L2 / Infinity Cache across the
+// segment's generators.  All rows staged (NST = 1 forward, 2 adjoint) is the LDS form.
+template <bool BACK, int NST>
+__global__ __launch_bounds__(NST == (BACK ? 2 : 1) ? 256 : 1024) void k_seg_row(double *__restrict__ l, double *__restrict__ r,
                                                              i64 nrows, i64 nel, const u64 *__restrict__ row_strs,
                                                              const u64 *__restrict__ el_strs, int row_is_alpha,
                                                              const SegGen *__restrict__ gens, int ng,
@@ -367,12 +368,12 @@ __global__ __launch_bounds__(GL ? 1024 : 256) void k_seg_row(double *__restrict_
     const u64 rs = row_strs[row];
     double *gl = l + row * nel;
     double *gr = BACK ? r + row * nel : nullptr;
-    double *xl = GL ? gl : reinterpret_cast<double *>(smem_raw);
-    double *xr = GL ? gr : xl + nel;
-    if (!GL) {
+    double *xl = NST >= 1 ? reinterpret_cast<double *>(smem_raw) : gl;
+    double *xr = BACK ? (NST >= 2 ? reinterpret_cast<double *>(smem_raw) + nel : gr) : nullptr;
+    if (NST > 0) {
         for (i64 t = tid; t < nel; t += bd) {
             xl[t] = gl[t];
-            if (BACK) xr[t] = gr[t];
+            if (BACK && NST >= 2) xr[t] = gr[t];
         }
         __syncthreads();
     }
@@ -404,21 +405,22 @@ __global__ __launch_bounds__(GL ? 1024 : 256) void k_seg_row(double *__restrict_
             const double t = block_sum(acc);
             if (tid == 0) partial[(i64)k * nrows + row] = t;
         }
-        __syncthreads();  // (GL: orders this generator's global updates before the next one's reads)
+        __syncthreads();  // (also orders this generator's global updates before the next one's reads)
     }
-    if (!GL) {
+    if (NST > 0) {
         for (i64 t = tid; t < nel; t += bd) {
             gl[t] = xl[t];
-            if (BACK) gr[t] = xr[t];
+            if (BACK && NST >= 2) gr[t] = xr[t];
         }
     }
 }
 
 // one block = one alpha pair (u_a, w_a) of the segment's alpha pattern; rows of NB elements
-// (the u and w strings of one pattern are disjoint, so blocks own disjoint row pairs).  GL: in
-// place in global memory, as in k_seg_row
-template <bool BACK, bool GL>
-__global__ __launch_bounds__(GL ? 1024 : 256) void k_seg_pair(double *__restrict__ l, double *__restrict__ r,
+// (the u and w strings of one pattern are disjoint, so blocks own disjoint row pairs).  NST of
+// the rows (l_u, l_w, then r_u, r_w) are staged in LDS, the others updated in place in global
+// memory, as in k_seg_row; NST = 2 forward / 4 adjoint is the LDS form
+template <bool BACK, int NST>
+__global__ __launch_bounds__(NST == (BACK ? 4 : 2) ? 256 : 1024) void k_seg_pair(double *__restrict__ l, double *__restrict__ r,
                                                               const int32_t *__restrict__ alist, i64 npairs, i64 NB,
                                                               const u64 *__restrict__ strsA,
                                                               const u64 *__restrict__ strsB,
@@ -430,24 +432,19 @@ __global__ __launch_bounds__(GL ? 1024 : 256) void k_seg_pair(double *__restrict
     const int tid = threadIdx.x, bd = blockDim.x;
     const int32_t ua = alist[2 * p], wa = alist[2 * p + 1];
     const u64 as = strsA[ua];
-    double *l0, *l1, *r0, *r1;
-    if (GL) {
-        l0 = l + (i64)ua * NB;
-        l1 = l + (i64)wa * NB;
-        r0 = BACK ? r + (i64)ua * NB : nullptr;
-        r1 = BACK ? r + (i64)wa * NB : nullptr;
-    } else {
-        l0 = reinterpret_cast<double *>(smem_raw);
-        l1 = l0 + NB;
-        r0 = l1 + NB;
-        r1 = r0 + NB;
+    double *const g0 = l + (i64)ua * NB, *const g1 = l + (i64)wa * NB;
+    double *const h0 = BACK ? r + (i64)ua * NB : nullptr, *const h1 = BACK ? r + (i64)wa * NB : nullptr;
+    double *const sm = reinterpret_cast<double *>(smem_raw);
+    double *l0 = NST >= 1 ? sm : g0;
+    double *l1 = NST >= 2 ? sm + NB : g1;
+    double *r0 = BACK ? (NST >= 3 ? sm + 2 * NB : h0) : nullptr;
+    double *r1 = BACK ? (NST >= 4 ? sm + 3 * NB : h1) : nullptr;
+    if (NST > 0) {
         for (i64 t = tid; t < NB; t += bd) {
-            l0[t] = l[(i64)ua * NB + t];
-            l1[t] = l[(i64)wa * NB + t];
-            if (BACK) {
-                r0[t] = r[(i64)ua * NB + t];
-                r1[t] = r[(i64)wa * NB + t];
-            }
+            l0[t] = g0[t];
+            if (NST >= 2) l1[t] = g1[t];
+            if (BACK && NST >= 3) r0[t] = h0[t];
+            if (BACK && NST >= 4) r1[t] = h1[t];
         }
         __syncthreads();
     }
@@ -478,16 +475,14 @@ __global__ __launch_bounds__(GL ? 1024 : 256) void k_seg_pair(double *__restrict
             const double t = block_sum(acc);
             if (tid == 0) partial[(i64)k * npairs + p] = t;
         }
-        __syncthreads();  // (GL: orders this generator's global updates before the next one's reads)
+        __syncthreads();  // (also orders this generator's global updates before the next one's reads)
     }
-    if (!GL) {
+    if (NST > 0) {
         for (i64 t = tid; t < NB; t += bd) {
-            l[(i64)ua * NB + t] = l0[t];
-            l[(i64)wa * NB + t] = l1[t];
-            if (BACK) {
-                r[(i64)ua * NB + t] = r0[t];
-                r[(i64)wa * NB + t] = r1[t];
-            }
+            g0[t] = l0[t];
+            if (NST >= 2) g1[t] = l1[t];
+            if (BACK && NST >= 3) h0[t] = r0[t];
+            if (BACK && NST >= 4) h1[t] = r1[t];
         }
     }
 }
@@ -521,7 +516,8 @@ __global__ __launch_bounds__(256) void k_transpose(const double *__restrict__ in
 
 struct Segment {
     int kind, first, count, pa;
-    bool glob;  // rows do not fit the LDS: the global-memory form of the segment kernel
+    int nst;    // rows per block staged in LDS (the rest updated in place in global memory)
+    int nrow;   // rows per block: 1 or 2 (ROW), 2 or 4 (PAIR), times 1 forward / 2 adjoint
 };
 
 static size_t seg_lds_budget() {
@@ -537,30 +533,35 @@ static std::vector<Segment> plan_sweep(DevPool *P, const i64 *idxs, i64 n, int n
     const DevBasis *B = P->basis;
     const i64 NA = B->pA->N, NB = B->pB->N;
     const size_t budget = seg_lds_budget();
-    // rows that do not fit the LDS are updated in place in global memory (segment.glob);
-    // sweep_mode 1 takes that form for every segment, 2 launches per generator
+    // as many of a block's rows as fit the LDS are staged there, the others updated in place
+    // in global memory; sweep_mode 1 stages none, 2 launches per generator, 3 stages one row
     const int mode = tuning(0).sweep_mode;
-    const bool rowA_ok = mode == 0 && (size_t)NB * nvec * 8 <= budget;
-    const bool rowB_ok = mode == 0 && (size_t)NA * nvec * 8 <= budget;
-    const bool pair_ok = mode == 0 && (size_t)NB * 2 * 2 * 8 <= budget;  // kernel reserves 4 rows
+    auto staged = [&](i64 len, int nrow) {
+        int k = mode == 1 ? 0 : (int)std::min<i64>(nrow, (i64)(budget / ((size_t)len * 8)));
+        if (mode == 3) k = std::min(k, 1);
+        return k == 3 ? 2 : k;  // (the pair kernel stages 0, 1, 2 or all 4 rows)
+    };
     std::vector<Segment> segs;
     for (i64 i = 0; i < n;) {
         const Generator &G = P->gens[idxs[i]];
-        Segment s{SEG_SINGLE, (int)i, 1, G.pa, false};
+        Segment s{SEG_SINGLE, (int)i, 1, G.pa, 0, 0};
         auto gen = [&](i64 j) -> const Generator & { return P->gens[idxs[j]]; };
         if (mode == 2) {
             // one launch per generator
         } else if (G.kind == 2) {
             s.kind = SEG_ROW_A;
-            s.glob = !rowA_ok;
+            s.nrow = nvec;
+            s.nst = staged(NB, s.nrow);
             while (i + s.count < n && gen(i + s.count).kind == 2) ++s.count;
         } else if (G.kind == 3 || (G.kind == 1 && i + 1 < n && gen(i + 1).kind == 3 && gen(i + 1).pa == G.pa)) {
             s.kind = SEG_PAIR;
-            s.glob = !pair_ok;
+            s.nrow = 2 * nvec;
+            s.nst = staged(NB, s.nrow);
             while (i + s.count < n && gen(i + s.count).pa == G.pa && gen(i + s.count).kind != 2) ++s.count;
         } else if (G.kind == 1) {
             s.kind = SEG_ROW_B;
-            s.glob = !rowB_ok;
+            s.nrow = nvec;
+            s.nst = staged(NA, s.nrow);
             while (i + s.count < n && gen(i + s.count).kind == 1 &&
                    !(i + s.count + 1 < n && gen(i + s.count + 1).kind == 3 &&
                      gen(i + s.count + 1).pa == gen(i + s.count).pa))
@@ -607,8 +608,9 @@ void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, doub
         if (n) HIPOTF_CHECK(hipMemcpyAsync(P->segdesc.p, host.data(), n * sizeof(SegGen), hipMemcpyHostToDevice, st));
         const SegGen *dg = P->segdesc.as<SegGen>();
         const int nvec = back ? 2 : 1;
-        // global-memory segments: LDS reserved only to hold one block per CU, so the rows the
-        // blocks in flight work on (256 CUs x up to 4 rows) stay within the L2 / Infinity Cache
+        // segments with rows in global memory: one block per CU (LDS reserved for that where no
+        // row is staged), so the rows the blocks in flight work on (256 CUs x up to 4 rows)
+        // stay within the L2 / Infinity Cache
         const size_t glob_smem = seg_lds_budget() / 2 + 4096;
         for (size_t si = 0; si < segs.size(); ++si) {
             const Segment &s = back ? segs[segs.size() - 1 - si] : segs[si];
@@ -634,8 +636,9 @@ void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, doub
                     l = P->tl.as<double>();
                     r = P->tr.as<double>();
                 }
-                const size_t smem = s.glob ? glob_smem : (size_t)nel * nvec * sizeof(double);
-                const int bd = s.glob ? 1024 : 256;
+                const bool full = s.nst == s.nrow;
+                const size_t smem = s.nst ? (size_t)nel * s.nst * sizeof(double) : glob_smem;
+                const int bd = full ? 256 : 1024;
                 double *part = nullptr;
                 if (back) {
                     P->segpart.ensure((size_t)s.count * nrows * sizeof(double));
@@ -650,11 +653,12 @@ void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, doub
                                        estr, A ? 1 : 0, dg + s.first, s.count, pp);
                 };
                 if (back) {
-                    if (s.glob) go_row(k_seg_row<true, true>, r, part);
-                    else go_row(k_seg_row<true, false>, r, part);
+                    if (s.nst == 0) go_row(k_seg_row<true, 0>, r, part);
+                    else if (s.nst == 1) go_row(k_seg_row<true, 1>, r, part);
+                    else go_row(k_seg_row<true, 2>, r, part);
                 } else {
-                    if (s.glob) go_row(k_seg_row<false, true>, (double *)nullptr, (double *)nullptr);
-                    else go_row(k_seg_row<false, false>, (double *)nullptr, (double *)nullptr);
+                    if (s.nst == 0) go_row(k_seg_row<false, 0>, (double *)nullptr, (double *)nullptr);
+                    else go_row(k_seg_row<false, 1>, (double *)nullptr, (double *)nullptr);
                 }
                 HIPOTF_CHECK(hipGetLastError());
                 if (!A) {
@@ -672,8 +676,9 @@ void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, doub
                     if (back) HIPOTF_CHECK(hipMemsetAsync(dres + s.first, 0, s.count * sizeof(double), st));
                     continue;
                 }
-                const size_t smem = s.glob ? glob_smem : (size_t)NB * 4 * sizeof(double);
-                const int bd = s.glob ? 1024 : 256;
+                const bool full = s.nst == s.nrow;
+                const size_t smem = s.nst ? (size_t)NB * s.nst * sizeof(double) : glob_smem;
+                const int bd = full ? 256 : 1024;
                 double *part = nullptr;
                 if (back) {
                     P->segpart.ensure((size_t)s.count * np * sizeof(double));
@@ -687,11 +692,14 @@ void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, doub
                                        s.count, pp);
                 };
                 if (back) {
-                    if (s.glob) go_pair(k_seg_pair<true, true>, lam, part);
-                    else go_pair(k_seg_pair<true, false>, lam, part);
+                    if (s.nst == 0) go_pair(k_seg_pair<true, 0>, lam, part);
+                    else if (s.nst == 1) go_pair(k_seg_pair<true, 1>, lam, part);
+                    else if (s.nst == 2) go_pair(k_seg_pair<true, 2>, lam, part);
+                    else go_pair(k_seg_pair<true, 4>, lam, part);
                 } else {
-                    if (s.glob) go_pair(k_seg_pair<false, true>, (double *)nullptr, (double *)nullptr);
-                    else go_pair(k_seg_pair<false, false>, (double *)nullptr, (double *)nullptr);
+                    if (s.nst == 0) go_pair(k_seg_pair<false, 0>, (double *)nullptr, (double *)nullptr);
+                    else if (s.nst == 1) go_pair(k_seg_pair<false, 1>, (double *)nullptr, (double *)nullptr);
+                    else go_pair(k_seg_pair<false, 2>, (double *)nullptr, (double *)nullptr);
                 }
                 HIPOTF_CHECK(hipGetLastError());
                 if (back)

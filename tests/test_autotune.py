@@ -37,7 +37,7 @@ def test_invalid_values_rejected_by_library_and_host(restore_tuning):
     before = AT.get_config(AT.FP64)
     for bad in ([4, 1, 1, 2, 1, 0, 8, 0], [0, 3, 1, 2, 1, 0, 8, 0], [0, 1, 0, 2, 1, 0, 8, 0],
                 [0, 1, 1, 3, 1, 0, 8, 0], [0, 1, 1, 2, 3, 0, 8, 0], [0, 1, 1, 2, 1, 2, 8, 0],
-                [0, 1, 1, 2, 1, 0, 2, 0], [0, 1, 1, 2, 1, 0, 8, 3]):
+                [0, 1, 1, 2, 1, 0, 2, 0], [0, 1, 1, 2, 1, 0, 8, 4]):
         v = np.array(bad, dtype=np.int32)
         assert lib.hipotf_set_tuning(0, v.ctypes.data_as(L._i32p), 8) != 0
         assert "tuning" in L.last_error()

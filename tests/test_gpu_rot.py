@@ -75,13 +75,14 @@ def test_fused_vqe_energy_grad(torch_cuda, n, na, nb):
     assert np.max(np.abs(host(psi) - lv)) < 1e-12  # reverse sweep uncomputes the state
 
 
-@pytest.mark.parametrize("sweep_mode", [0, 1, 2])
+@pytest.mark.parametrize("sweep_mode", [0, 1, 2, 3])
 def test_vqe_config2_matches_reference_fixture(torch_cuda, sweep_mode):
     """12 orbitals, 6+6 electrons, all 1,818 UCC parameters, theta ~ U[-0.05, 0.05]: the fused
     energy + adjoint gradient equal the values the compiled reference produced (golden), for
     each sweep form: fused segments with rows in LDS (0; everything fits at this size), fused
-    segments updating the rows in global memory (1; the form larger systems take), and one
-    launch per generator (2)."""
+    segments updating the rows in global memory (1), one launch per generator (2), and
+    segments with one row per block in LDS and the others in global memory (3; the form
+    16 orbitals takes)."""
     import os
     from binsim_amd import autotune as AT
     f = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config0_n12.npz"))
