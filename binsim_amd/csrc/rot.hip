@@ -356,7 +356,7 @@ enum { SEG_ROW_A = 0, SEG_ROW_B = 1, SEG_PAIR = 2, SEG_SINGLE = 3 };
 // one block per CU so that the rows in flight stay in the L2 / Infinity Cache across the
 // segment's generators.  All rows staged (NST = 1 forward, 2 adjoint) is the LDS form.
 template <bool BACK, int NST>
-__global__ __launch_bounds__(NST == (BACK ? 2 : 1) ? 256 : 1024) void k_seg_row(double *__restrict__ l, double *__restrict__ r,
+__global__ __launch_bounds__(1024) void k_seg_row(double *__restrict__ l, double *__restrict__ r,
                                                              i64 nrows, i64 nel, const u64 *__restrict__ row_strs,
                                                              const u64 *__restrict__ el_strs, int row_is_alpha,
                                                              const SegGen *__restrict__ gens, int ng,
@@ -420,7 +420,7 @@ __global__ __launch_bounds__(NST == (BACK ? 2 : 1) ? 256 : 1024) void k_seg_row(
 // the rows (l_u, l_w, then r_u, r_w) are staged in LDS, the others updated in place in global
 // memory, as in k_seg_row; NST = 2 forward / 4 adjoint is the LDS form
 template <bool BACK, int NST>
-__global__ __launch_bounds__(NST == (BACK ? 4 : 2) ? 256 : 1024) void k_seg_pair(double *__restrict__ l, double *__restrict__ r,
+__global__ __launch_bounds__(1024) void k_seg_pair(double *__restrict__ l, double *__restrict__ r,
                                                               const int32_t *__restrict__ alist, i64 npairs, i64 NB,
                                                               const u64 *__restrict__ strsA,
                                                               const u64 *__restrict__ strsB,
@@ -636,9 +636,10 @@ void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, doub
                     l = P->tl.as<double>();
                     r = P->tr.as<double>();
                 }
-                const bool full = s.nst == s.nrow;
                 const size_t smem = s.nst ? (size_t)nel * s.nst * sizeof(double) : glob_smem;
-                const int bd = full ? 256 : 1024;
+                // 1024 threads where the LDS holds one block per CU (latency of the per-generator
+                // pair-list and string loads), 256 where several blocks share a CU
+                const int bd = smem > glob_smem / 2 ? 1024 : 256;
                 double *part = nullptr;
                 if (back) {
                     P->segpart.ensure((size_t)s.count * nrows * sizeof(double));
@@ -676,9 +677,8 @@ void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, doub
                     if (back) HIPOTF_CHECK(hipMemsetAsync(dres + s.first, 0, s.count * sizeof(double), st));
                     continue;
                 }
-                const bool full = s.nst == s.nrow;
                 const size_t smem = s.nst ? (size_t)NB * s.nst * sizeof(double) : glob_smem;
-                const int bd = full ? 256 : 1024;
+                const int bd = smem > glob_smem / 2 ? 1024 : 256;
                 double *part = nullptr;
                 if (back) {
                     P->segpart.ensure((size_t)s.count * np * sizeof(double));
