@@ -205,9 +205,10 @@ struct SigmaTuning {
     int rs_block = 8;       // row SpMM: destination rows sharing each gathered source row
                             // (4, 8, 16), or 0 for one row per wave
     int sweep_mode = 0;     // VQE rotation sweeps (FP64 table): 0 fused segments with as many
-                            // rows in LDS as fit, the others in global memory; 1 fused, rows
+                            // rows (and the leading part of one more) in LDS as fit, the
+                            // others in global memory; 1 fused, rows
                             // always in global memory; 2 one launch per generator; 3 fused,
-                            // at most one row per block in LDS
+                            // at most one row and half of the next per block in LDS
 };
 SigmaTuning &tuning(int precision);  // 0: FP64 kernels, 1: FP32 kernels
 constexpr int TUNING_FIELDS = 8;

@@ -350,8 +350,30 @@ struct SegGen {
 
 enum { SEG_ROW_A = 0, SEG_ROW_B = 1, SEG_PAIR = 2, SEG_SINGLE = 3 };
 
+// A row of a segment block: in LDS (M = 0), in global memory (M = 2), or split (M = 1): its
+// first sp elements staged in LDS, the rest in global memory
+template <int M>
+struct SegRow {
+    double *s, *g;
+    i64 sp;
+    __device__ __forceinline__ double ld(i64 i) const {
+        if constexpr (M == 0) return s[i];
+        else if constexpr (M == 2) return g[i];
+        else return i < sp ? s[i] : g[i];
+    }
+    __device__ __forceinline__ void st(i64 i, double v) const {
+        if constexpr (M == 0) s[i] = v;
+        else if constexpr (M == 2) g[i] = v;
+        else if (i < sp) s[i] = v;
+        else g[i] = v;
+    }
+};
+// row k of a block with NST fully staged rows: staged, split (the next one), or global
+constexpr int seg_mode(int k, int NST) { return k < NST ? 0 : (k == NST ? 1 : 2); }
+
 // one block = one row of nel elements; BACK: adjoint step on (l, r) with gradient partials.
-// NST of the block's rows (l, then r) are staged in LDS; rows that do not fit stay in global
+// NST of the block's rows (l, then r) are staged in LDS, the next one's first sp elements too
+// (rows are staged in that order, row k at LDS offset k * nel); rows that do not fit stay in global
 // memory and are updated in place, one generator after the other, with the launch sized to
 // one block per CU so that the rows in flight stay in the L2 / Infinity Cache across the
 // segment's generators.  All rows staged (NST = 1 forward, 2 adjoint) is the LDS form.
@@ -360,7 +382,7 @@ __global__ __launch_bounds__(1024) void k_seg_row(double *__restrict__ l, double
                                                              i64 nrows, i64 nel, const u64 *__restrict__ row_strs,
                                                              const u64 *__restrict__ el_strs, int row_is_alpha,
                                                              const SegGen *__restrict__ gens, int ng,
-                                                             double *__restrict__ partial) {
+                                                             double *__restrict__ partial, i64 sp) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const i64 row = blockIdx.x;
     if (row >= nrows) return;
@@ -368,15 +390,22 @@ __global__ __launch_bounds__(1024) void k_seg_row(double *__restrict__ l, double
     const u64 rs = row_strs[row];
     double *gl = l + row * nel;
     double *gr = BACK ? r + row * nel : nullptr;
-    double *xl = NST >= 1 ? reinterpret_cast<double *>(smem_raw) : gl;
-    double *xr = BACK ? (NST >= 2 ? reinterpret_cast<double *>(smem_raw) + nel : gr) : nullptr;
-    if (NST > 0) {
-        for (i64 t = tid; t < nel; t += bd) {
-            xl[t] = gl[t];
-            if (BACK && NST >= 2) xr[t] = gr[t];
+    double *const sm = reinterpret_cast<double *>(smem_raw);
+    const SegRow<seg_mode(0, NST)> xl{sm, gl, NST == 0 ? sp : nel};
+    const SegRow<seg_mode(1, NST)> xr{sm + nel, gr, sp};
+    // stage: rows below NST whole, row NST up to sp
+    auto stage = [&](bool in) {
+        for (int k = 0; k < (BACK ? 2 : 1); ++k) {
+            const i64 m = k < NST ? nel : (k == NST ? sp : 0);
+            double *g = k == 0 ? gl : gr, *sk = sm + (i64)k * nel;
+            for (i64 t = tid; t < m; t += bd) {
+                if (in) sk[t] = g[t];
+                else g[t] = sk[t];
+            }
         }
-        __syncthreads();
-    }
+    };
+    stage(true);
+    __syncthreads();
     for (int kk = 0; kk < ng; ++kk) {
         const int k = BACK ? ng - 1 - kk : kk;
         const SegGen sg = gens[k];
@@ -387,18 +416,18 @@ __global__ __launch_bounds__(1024) void k_seg_row(double *__restrict__ l, double
             const bool neg = row_is_alpha ? jw_sign(sg.g, rs, es) : jw_sign(sg.g, es, rs);
             const double ts = neg ? -sg.s : sg.s;
             if (!BACK) {
-                const double xu = xl[u], xw = xl[w];
-                xl[u] = sg.c * xu - ts * xw;
-                xl[w] = ts * xu + sg.c * xw;
+                const double xu = xl.ld(u), xw = xl.ld(w);
+                xl.st(u, sg.c * xu - ts * xw);
+                xl.st(w, ts * xu + sg.c * xw);
             } else {
                 const double tc = neg ? -sg.c : sg.c;
-                const double lu = xl[u], lw = xl[w], ru = xr[u], rw = xr[w];
+                const double lu = xl.ld(u), lw = xl.ld(w), ru = xr.ld(u), rw = xr.ld(w);
                 const double nlu = sg.c * lu + ts * lw, nlw = -ts * lu + sg.c * lw;
                 acc += nlu * (-sg.s * ru + tc * rw) + nlw * (-sg.s * rw - tc * ru);
-                xl[u] = nlu;
-                xl[w] = nlw;
-                xr[u] = sg.c * ru + ts * rw;
-                xr[w] = -ts * ru + sg.c * rw;
+                xl.st(u, nlu);
+                xl.st(w, nlw);
+                xr.st(u, sg.c * ru + ts * rw);
+                xr.st(w, -ts * ru + sg.c * rw);
             }
         }
         if (BACK) {
@@ -407,12 +436,7 @@ __global__ __launch_bounds__(1024) void k_seg_row(double *__restrict__ l, double
         }
         __syncthreads();  // (also orders this generator's global updates before the next one's reads)
     }
-    if (NST > 0) {
-        for (i64 t = tid; t < nel; t += bd) {
-            gl[t] = xl[t];
-            if (BACK && NST >= 2) gr[t] = xr[t];
-        }
-    }
+    stage(false);
 }
 
 // one block = one alpha pair (u_a, w_a) of the segment's alpha pattern; rows of NB elements
@@ -425,7 +449,7 @@ __global__ __launch_bounds__(1024) void k_seg_pair(double *__restrict__ l, doubl
                                                               const u64 *__restrict__ strsA,
                                                               const u64 *__restrict__ strsB,
                                                               const SegGen *__restrict__ gens, int ng,
-                                                              double *__restrict__ partial) {
+                                                              double *__restrict__ partial, i64 sp) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const i64 p = blockIdx.x;
     if (p >= npairs) return;
@@ -435,19 +459,22 @@ __global__ __launch_bounds__(1024) void k_seg_pair(double *__restrict__ l, doubl
     double *const g0 = l + (i64)ua * NB, *const g1 = l + (i64)wa * NB;
     double *const h0 = BACK ? r + (i64)ua * NB : nullptr, *const h1 = BACK ? r + (i64)wa * NB : nullptr;
     double *const sm = reinterpret_cast<double *>(smem_raw);
-    double *l0 = NST >= 1 ? sm : g0;
-    double *l1 = NST >= 2 ? sm + NB : g1;
-    double *r0 = BACK ? (NST >= 3 ? sm + 2 * NB : h0) : nullptr;
-    double *r1 = BACK ? (NST >= 4 ? sm + 3 * NB : h1) : nullptr;
-    if (NST > 0) {
-        for (i64 t = tid; t < NB; t += bd) {
-            l0[t] = g0[t];
-            if (NST >= 2) l1[t] = g1[t];
-            if (BACK && NST >= 3) r0[t] = h0[t];
-            if (BACK && NST >= 4) r1[t] = h1[t];
+    const SegRow<seg_mode(0, NST)> l0{sm, g0, NST == 0 ? sp : NB};
+    const SegRow<seg_mode(1, NST)> l1{sm + NB, g1, NST == 1 ? sp : NB};
+    const SegRow<seg_mode(2, NST)> r0{sm + 2 * NB, h0, NST == 2 ? sp : NB};
+    const SegRow<seg_mode(3, NST)> r1{sm + 3 * NB, h1, sp};
+    auto stage = [&](bool in) {
+        for (int k = 0; k < (BACK ? 4 : 2); ++k) {
+            const i64 m = k < NST ? NB : (k == NST ? sp : 0);
+            double *g = k == 0 ? g0 : (k == 1 ? g1 : (k == 2 ? h0 : h1)), *sk = sm + (i64)k * NB;
+            for (i64 t = tid; t < m; t += bd) {
+                if (in) sk[t] = g[t];
+                else g[t] = sk[t];
+            }
         }
-        __syncthreads();
-    }
+    };
+    stage(true);
+    __syncthreads();
     for (int kk = 0; kk < ng; ++kk) {
         const int k = BACK ? ng - 1 - kk : kk;
         const SegGen sg = gens[k];
@@ -457,18 +484,18 @@ __global__ __launch_bounds__(1024) void k_seg_pair(double *__restrict__ l, doubl
             const bool neg = jw_sign(sg.g, as, strsB[ub]);
             const double ts = neg ? -sg.s : sg.s;
             if (!BACK) {
-                const double xu = l0[ub], xw = l1[wb];
-                l0[ub] = sg.c * xu - ts * xw;
-                l1[wb] = ts * xu + sg.c * xw;
+                const double xu = l0.ld(ub), xw = l1.ld(wb);
+                l0.st(ub, sg.c * xu - ts * xw);
+                l1.st(wb, ts * xu + sg.c * xw);
             } else {
                 const double tc = neg ? -sg.c : sg.c;
-                const double lu = l0[ub], lw = l1[wb], ru = r0[ub], rw = r1[wb];
+                const double lu = l0.ld(ub), lw = l1.ld(wb), ru = r0.ld(ub), rw = r1.ld(wb);
                 const double nlu = sg.c * lu + ts * lw, nlw = -ts * lu + sg.c * lw;
                 acc += nlu * (-sg.s * ru + tc * rw) + nlw * (-sg.s * rw - tc * ru);
-                l0[ub] = nlu;
-                l1[wb] = nlw;
-                r0[ub] = sg.c * ru + ts * rw;
-                r1[wb] = -ts * ru + sg.c * rw;
+                l0.st(ub, nlu);
+                l1.st(wb, nlw);
+                r0.st(ub, sg.c * ru + ts * rw);
+                r1.st(wb, -ts * ru + sg.c * rw);
             }
         }
         if (BACK) {
@@ -477,14 +504,7 @@ __global__ __launch_bounds__(1024) void k_seg_pair(double *__restrict__ l, doubl
         }
         __syncthreads();  // (also orders this generator's global updates before the next one's reads)
     }
-    if (NST > 0) {
-        for (i64 t = tid; t < NB; t += bd) {
-            g0[t] = l0[t];
-            if (NST >= 2) g1[t] = l1[t];
-            if (BACK && NST >= 3) h0[t] = r0[t];
-            if (BACK && NST >= 4) h1[t] = r1[t];
-        }
-    }
+    stage(false);
 }
 
 // dres[k] = sum_b partial[k * nb + b], one block per generator of the segment
@@ -516,8 +536,9 @@ __global__ __launch_bounds__(256) void k_transpose(const double *__restrict__ in
 
 struct Segment {
     int kind, first, count, pa;
-    int nst;    // rows per block staged in LDS (the rest updated in place in global memory)
-    int nrow;   // rows per block: 1 or 2 (ROW), 2 or 4 (PAIR), times 1 forward / 2 adjoint
+    int nst;    // rows per block staged in LDS whole (the rest updated in place in global memory)
+    i64 sp;     // elements of the next row staged too
+    int nrow;   // rows per block: 1 (ROW) or 2 (PAIR), times 1 forward / 2 adjoint
 };
 
 static size_t seg_lds_budget() {
@@ -535,33 +556,42 @@ static std::vector<Segment> plan_sweep(DevPool *P, const i64 *idxs, i64 n, int n
     const size_t budget = seg_lds_budget();
     // as many of a block's rows as fit the LDS are staged there, the others updated in place
     // in global memory; sweep_mode 1 stages none, 2 launches per generator, 3 stages one row
+    // and half of the next
     const int mode = tuning(0).sweep_mode;
-    auto staged = [&](i64 len, int nrow) {
-        int k = mode == 1 ? 0 : (int)std::min<i64>(nrow, (i64)(budget / ((size_t)len * 8)));
-        if (mode == 3) k = std::min(k, 1);
-        return k == 3 ? 2 : k;  // (the pair kernel stages 0, 1, 2 or all 4 rows)
+    // whole rows, then the part of the next row that fits (at least 1024 elements); mode 3 at
+    // most one whole row and half of the next
+    auto stage = [&](Segment &s, i64 len) {
+        const i64 cap = mode == 1 ? 0 : (i64)(budget / 8);
+        s.nst = (int)std::min<i64>(s.nrow, cap / len);
+        s.sp = 0;
+        if (mode == 3) {
+            s.nst = std::min(s.nst, 1);
+            if (s.nst < s.nrow && cap - s.nst * len >= len / 2) s.sp = len / 2;
+        } else if (s.nst < s.nrow && cap - s.nst * len >= 1024) {
+            s.sp = cap - s.nst * len;
+        }
     };
     std::vector<Segment> segs;
     for (i64 i = 0; i < n;) {
         const Generator &G = P->gens[idxs[i]];
-        Segment s{SEG_SINGLE, (int)i, 1, G.pa, 0, 0};
+        Segment s{SEG_SINGLE, (int)i, 1, G.pa, 0, 0, 0};
         auto gen = [&](i64 j) -> const Generator & { return P->gens[idxs[j]]; };
         if (mode == 2) {
             // one launch per generator
         } else if (G.kind == 2) {
             s.kind = SEG_ROW_A;
             s.nrow = nvec;
-            s.nst = staged(NB, s.nrow);
+            stage(s, NB);
             while (i + s.count < n && gen(i + s.count).kind == 2) ++s.count;
         } else if (G.kind == 3 || (G.kind == 1 && i + 1 < n && gen(i + 1).kind == 3 && gen(i + 1).pa == G.pa)) {
             s.kind = SEG_PAIR;
             s.nrow = 2 * nvec;
-            s.nst = staged(NB, s.nrow);
+            stage(s, NB);
             while (i + s.count < n && gen(i + s.count).pa == G.pa && gen(i + s.count).kind != 2) ++s.count;
         } else if (G.kind == 1) {
             s.kind = SEG_ROW_B;
             s.nrow = nvec;
-            s.nst = staged(NA, s.nrow);
+            stage(s, NA);
             while (i + s.count < n && gen(i + s.count).kind == 1 &&
                    !(i + s.count + 1 < n && gen(i + s.count + 1).kind == 3 &&
                      gen(i + s.count + 1).pa == gen(i + s.count).pa))
@@ -636,7 +666,8 @@ void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, doub
                     l = P->tl.as<double>();
                     r = P->tr.as<double>();
                 }
-                const size_t smem = s.nst ? (size_t)nel * s.nst * sizeof(double) : glob_smem;
+                const size_t staged = (size_t)(nel * s.nst + s.sp) * sizeof(double);
+                const size_t smem = staged ? staged : glob_smem;
                 // 1024 threads where the LDS holds one block per CU (latency of the per-generator
                 // pair-list and string loads), 256 where several blocks share a CU
                 const int bd = smem > glob_smem / 2 ? 1024 : 256;
@@ -651,7 +682,7 @@ void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, doub
                     HIPOTF_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                                      (int)smem));
                     hipLaunchKernelGGL(kern, dim3((unsigned)nrows), dim3(bd), smem, st, l, rr, nrows, nel, rstr,
-                                       estr, A ? 1 : 0, dg + s.first, s.count, pp);
+                                       estr, A ? 1 : 0, dg + s.first, s.count, pp, s.sp);
                 };
                 if (back) {
                     if (s.nst == 0) go_row(k_seg_row<true, 0>, r, part);
@@ -677,7 +708,8 @@ void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, doub
                     if (back) HIPOTF_CHECK(hipMemsetAsync(dres + s.first, 0, s.count * sizeof(double), st));
                     continue;
                 }
-                const size_t smem = s.nst ? (size_t)NB * s.nst * sizeof(double) : glob_smem;
+                const size_t staged = (size_t)(NB * s.nst + s.sp) * sizeof(double);
+                const size_t smem = staged ? staged : glob_smem;
                 const int bd = smem > glob_smem / 2 ? 1024 : 256;
                 double *part = nullptr;
                 if (back) {
@@ -689,12 +721,13 @@ void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, doub
                                                      (int)smem));
                     hipLaunchKernelGGL(kern, dim3((unsigned)np), dim3(bd), smem, st, psi, rr, LA.uw.as<int32_t>(),
                                        np, NB, B->pA->strs.as<u64>(), B->pB->strs.as<u64>(), dg + s.first,
-                                       s.count, pp);
+                                       s.count, pp, s.sp);
                 };
                 if (back) {
                     if (s.nst == 0) go_pair(k_seg_pair<true, 0>, lam, part);
                     else if (s.nst == 1) go_pair(k_seg_pair<true, 1>, lam, part);
                     else if (s.nst == 2) go_pair(k_seg_pair<true, 2>, lam, part);
+                    else if (s.nst == 3) go_pair(k_seg_pair<true, 3>, lam, part);
                     else go_pair(k_seg_pair<true, 4>, lam, part);
                 } else {
                     if (s.nst == 0) go_pair(k_seg_pair<false, 0>, (double *)nullptr, (double *)nullptr);

@@ -81,8 +81,8 @@ def test_vqe_config2_matches_reference_fixture(torch_cuda, sweep_mode):
     energy + adjoint gradient equal the values the compiled reference produced (golden), for
     each sweep form: fused segments with rows in LDS (0; everything fits at this size), fused
     segments updating the rows in global memory (1), one launch per generator (2), and
-    segments with one row per block in LDS and the others in global memory (3; the form
-    16 orbitals takes)."""
+    segments with one row and half of the next per block in LDS and the rest in global memory
+    (3; the forms 16 orbitals takes: whole, split and global rows)."""
     import os
     from binsim_amd import autotune as AT
     f = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config0_n12.npz"))
