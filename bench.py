@@ -104,9 +104,14 @@ def fci_block(torch, B, H, nroots, maxspace, tol, maxiter):
     return out
 
 
-def vqe(torch, n, na, nb, nfd, seed_off=2000, amp=0.05):
+def vqe(torch, n, na, nb, nfd, seed_off=2000, amp=0.05, stored_tuning=False):
     import numpy as np
     B, H, P, F = build(n, na, nb, pool=True)
+    if stored_tuning:
+        from binsim_amd import autotune as AT
+        entry = AT.lookup(B)
+        if entry is not None:
+            AT.apply(entry)
     from binsim_amd.ansatz import ucc_spin_orbitals
     from binsim_amd.integrals import SEED_BASE
     npar = len(ucc_spin_orbitals(n, na, nb))
@@ -201,7 +206,8 @@ def main():
     assert torch.cuda.is_available(), "no GPU visible"
     if args.vqe_only:
         m = args.vqe_only
-        r = vqe(torch, m, m // 2, m // 2, nfd=0, seed_off=2000, amp=0.05 if m == 12 else 0.02)
+        r = vqe(torch, m, m // 2, m // 2, nfd=0, seed_off=2000, amp=0.05 if m == 12 else 0.02,
+                stored_tuning=True)
         print(json.dumps({"vqe": m, **r}))
         return
     n, na, nb = 16, 8, 8
@@ -282,6 +288,9 @@ def main():
         del B0, H0
         result["vqe_12o"] = vqe(torch, 12, 6, 6, nfd=8)
         torch.cuda.empty_cache()
+        # the 16-orbital launch choices again (the 12-orbital ones differ)
+        AT.set_config(AT.FP64, tuning["fp64"])
+        AT.set_config(AT.FP32, tuning["fp32"])
         result["vqe_16o"] = vqe(torch, 16, 8, 8, nfd=2, seed_off=2000, amp=0.02)
     if not args.no_cpu:
         cb = cpu_baseline()
