@@ -104,6 +104,29 @@ def fci_block(torch, B, H, nroots, maxspace, tol, maxiter):
     return out
 
 
+def fci16_ground(torch, maxiter):
+    """16-orbital 8a8b ground state to 1e-6 from the lowest diagonal: plain FP64 (the reference's
+    restart-to-3 Davidson) and the mixed solver (block 1, maxspace 32), launch choices from the
+    stored tuning entry."""
+    from binsim_amd import autotune as AT
+    from binsim_amd import davidson as D
+    B, H, _, _ = build(16, 8, 8)
+    entry = AT.lookup(B)
+    if entry is not None:
+        AT.apply(entry)
+    out = {"fci16": "ground state", "tol": 1e-6, "maxiter": maxiter}
+    for name, mixed, block, maxspace in (("plain_fp64", False, 0, 0), ("mixed_block1", True, 1, 32)):
+        e, _, info = D.solve(B, H, nroots=1, mixed=mixed, tol=1e-6, block=block, maxspace=maxspace,
+                             maxiter=maxiter, verbose=True)
+        out[name] = dict(seconds=round(info["seconds"], 3), iterations=int(info["iterations"]),
+                         sigma_fp64=int(info["sigma64"]), sigma_fp32=int(info["sigma32"]),
+                         converged=bool(info["converged"] == 1), energy=float(e[0]))
+        log(f"fci16 {name}: {info['seconds']:.2f} s, {int(info['iterations'])} it, e = {e[0]}")
+    out["abs_dE"] = abs(out["plain_fp64"]["energy"] - out["mixed_block1"]["energy"])
+    out["speedup_mixed_vs_plain"] = round(out["plain_fp64"]["seconds"] / out["mixed_block1"]["seconds"], 3)
+    return out
+
+
 def vqe(torch, n, na, nb, nfd, seed_off=2000, amp=0.05, stored_tuning=False):
     import numpy as np
     B, H, P, F = build(n, na, nb, pool=True)
@@ -194,6 +217,9 @@ def main():
                     help="one FP64 sigma and one plain copy of the same size (counter runs)")
     ap.add_argument("--vqe-only", type=int, choices=(12, 16), default=None,
                     help="only the VQE energy + adjoint gradient at 12 or 16 orbitals (profiling runs)")
+    ap.add_argument("--davidson-only", type=int, default=None, metavar="MAXITER",
+                    help="only the 16-orbital ground state to 1e-6: plain FP64, then mixed (block 1, "
+                         "maxspace 32), each stopped after MAXITER iterations (600: to convergence)")
     ap.add_argument("--tune", action="store_true",
                     help="run the sigma autotuner on the workload first (about a minute) instead "
                          "of applying the stored choice")
@@ -209,6 +235,9 @@ def main():
         r = vqe(torch, m, m // 2, m // 2, nfd=0, seed_off=2000, amp=0.05 if m == 12 else 0.02,
                 stored_tuning=True)
         print(json.dumps({"vqe": m, **r}))
+        return
+    if args.davidson_only:
+        print(json.dumps(fci16_ground(torch, args.davidson_only)))
         return
     n, na, nb = 16, 8, 8
     t = time.perf_counter()

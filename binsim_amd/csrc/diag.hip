@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 namespace hipotf {
 
@@ -33,23 +34,73 @@ __device__ __forceinline__ double block_sum_d(double v) {
 
 static int nblk(i64 n) { return (int)std::max<i64>(1, std::min<i64>((n + 255) / 256, 2048)); }
 
-// y = sum_j c_j V_j ; if AV given: r = sum_j c_j AV_j - e y (written to y), partial |r|^2
-__global__ __launch_bounds__(256) void k_combine(VecPack V, VecPack AV, int dim, int with_res,
-                                                 double e, double *__restrict__ out,
+// f(std::integral_constant<int, c>) for c = k in 1..8 (k > 8 -> 8)
+template <typename F>
+static void dispatch_m8(int k, F &&f) {
+    switch (k) {
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 3: f(std::integral_constant<int, 3>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 5: f(std::integral_constant<int, 5>{}); break;
+    case 6: f(std::integral_constant<int, 6>{}); break;
+    case 7: f(std::integral_constant<int, 7>{}); break;
+    default: f(std::integral_constant<int, 8>{}); break;
+    }
+}
+
+// Streaming kernels keep several independent loads in flight per thread: the vector loops load
+// 4 vectors' elements before using them, the single-vector loops 4 elements (256 apart) per step.
+constexpr int VU = 4;
+
+// y = sum_j c_j V_j ; with RES: r = sum_j c_j AV_j - e y (written to y), partial |r|^2
+// (the sum runs over j in order, as before the loads were batched)
+template <bool RES>
+__global__ __launch_bounds__(256) void k_combine(VecPack V, VecPack AV, int dim, double e, double *__restrict__ out,
                                                  double *__restrict__ partial, i64 n) {
     double acc = 0.0;
     for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
         double x = 0.0, ax = 0.0;
-        for (int j = 0; j < dim; ++j) {
-            x += V.c[j] * V.v[j][i];
-            if (with_res) ax += V.c[j] * AV.v[j][i];
+        int j = 0;
+        for (; j + VU <= dim; j += VU) {
+            double xv[VU], av[VU];
+#pragma unroll
+            for (int u = 0; u < VU; ++u) {
+                xv[u] = V.v[j + u][i];
+                if (RES) av[u] = AV.v[j + u][i];
+            }
+#pragma unroll
+            for (int u = 0; u < VU; ++u) {
+                x += V.c[j + u] * xv[u];
+                if (RES) ax += V.c[j + u] * av[u];
+            }
         }
-        const double r = with_res ? ax - e * x : x;
+        for (; j < dim; ++j) {
+            x += V.c[j] * V.v[j][i];
+            if (RES) ax += V.c[j] * AV.v[j][i];
+        }
+        const double r = RES ? ax - e * x : x;
         out[i] = r;
         acc += r * r;
     }
     const double t = block_sum_d(acc);
     if (threadIdx.x == 0 && partial) partial[blockIdx.x] = t;
+}
+
+// Calls f(i) for every i < n, VU indices per thread and step: the full steps are unrolled so
+// their loads issue together, the last partial step is checked per index.
+template <typename F>
+__device__ __forceinline__ void stream_for(i64 n, F &&f) {
+    const i64 step = (i64)gridDim.x * blockDim.x * VU;
+    for (i64 base = (i64)blockIdx.x * blockDim.x * VU + threadIdx.x; base < n; base += step) {
+        if (base + (i64)(VU - 1) * blockDim.x < n) {
+#pragma unroll
+            for (int u = 0; u < VU; ++u) f(base + (i64)u * blockDim.x);
+        } else {
+            for (int u = 0; u < VU; ++u)
+                if (base + (i64)u * blockDim.x < n) f(base + (i64)u * blockDim.x);
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ partial, int n,
@@ -63,18 +114,17 @@ __global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ parti
 __global__ __launch_bounds__(256) void k_precond(double *__restrict__ axt, const double *__restrict__ diags,
                                                  double e, double shift, i64 n) {
     const double as = fabs(shift);
-    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
+    stream_for(n, [&](i64 i) {
         double d = diags[i] - e;
         if (fabs(d) < as) d = d >= 0 ? as : -as;
         axt[i] = axt[i] / d;
-    }
+    });
 }
 
 __global__ __launch_bounds__(256) void k_dotp(const double *__restrict__ a, const double *__restrict__ b,
                                               double *__restrict__ partial, i64 n) {
     double acc = 0.0;
-    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
-        acc += a[i] * b[i];
+    stream_for(n, [&](i64 i) { acc += a[i] * b[i]; });
     const double t = block_sum_d(acc);
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
@@ -83,29 +133,33 @@ __global__ __launch_bounds__(256) void k_dotp(const double *__restrict__ a, cons
 __global__ __launch_bounds__(256) void k_axpy_dev(double *__restrict__ y, const double *__restrict__ x,
                                                   const double *__restrict__ proj, i64 n) {
     const double p = *proj;
-    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
-        y[i] -= p * x[i];
+    stream_for(n, [&](i64 i) { y[i] -= p * x[i]; });
 }
 
 // y *= 1 / *nrm
 __global__ __launch_bounds__(256) void k_scale_inv(double *__restrict__ y, const double *__restrict__ nrm,
                                                    i64 n) {
     const double s = 1.0 / *nrm;
-    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
-        y[i] *= s;
+    stream_for(n, [&](i64 i) { y[i] *= s; });
 }
 
-// out[j] = <V_j, a> for j < m (m <= 8): each block accumulates 8 partials
-__global__ __launch_bounds__(256) void k_multidot(VecPack V, int m, const double *__restrict__ a,
+// out[j] = <V_j, a> for j < M (M <= 8): each block accumulates M partials
+template <int M>
+__global__ __launch_bounds__(256) void k_multidot(VecPack V, const double *__restrict__ a,
                                                   double *__restrict__ partial, int ldp, i64 n) {
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double acc[M];
+#pragma unroll
+    for (int j = 0; j < M; ++j) acc[j] = 0.0;
     for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
         const double av = a[i];
+        double v[M];
 #pragma unroll
-        for (int j = 0; j < 8; ++j)
-            if (j < m) acc[j] += V.v[j][i] * av;
+        for (int j = 0; j < M; ++j) v[j] = V.v[j][i];
+#pragma unroll
+        for (int j = 0; j < M; ++j) acc[j] += v[j] * av;
     }
-    for (int j = 0; j < m; ++j) {
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
         const double t = block_sum_d(acc[j]);
         if (threadIdx.x == 0) partial[j * ldp + blockIdx.x] = t;
     }
@@ -148,7 +202,7 @@ double ritz_and_residual(i64 N, int dim, const double *coeffs, double min_e, con
     pack(Q, AV, coeffs, dim);
     auto &S = scratch();
     const int nb = nblk(N);
-    hipLaunchKernelGGL(k_combine, dim3(nb), dim3(256), 0, st, P, Q, dim, 1, min_e, axt,
+    hipLaunchKernelGGL(k_combine<true>, dim3(nb), dim3(256), 0, st, P, Q, dim, min_e, axt,
                        S.partial.as<double>(), N);
     HIPOTF_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, st, S.partial.as<double>(), N > 0 ? nb : 0,
@@ -161,7 +215,7 @@ void ritz_vector(i64 N, int dim, const double *coeffs, const double *const *V, d
                  hipStream_t st) {
     VecPack P{}, Q{};
     pack(P, V, coeffs, dim);
-    hipLaunchKernelGGL(k_combine, dim3(nblk(N)), dim3(256), 0, st, P, Q, dim, 0, 0.0, xt,
+    hipLaunchKernelGGL(k_combine<false>, dim3(nblk(N)), dim3(256), 0, st, P, Q, dim, 0.0, xt,
                        (double *)nullptr, N);
     HIPOTF_CHECK(hipGetLastError());
 }
@@ -227,7 +281,10 @@ void multidot(i64 N, int dim, const double *const *V, const double *a, double *o
         const int m = std::min(8, dim - j0);
         VecPack P{};
         pack(P, V + j0, nullptr, m);
-        hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(256), 0, st, P, m, a, S.partial.as<double>(), 2048, N);
+        dispatch_m8(m, [&](auto mc) {
+            hipLaunchKernelGGL(k_multidot<decltype(mc)::value>, dim3(nb), dim3(256), 0, st, P, a,
+                               S.partial.as<double>(), 2048, N);
+        });
         HIPOTF_CHECK(hipGetLastError());
         for (int j = 0; j < m; ++j) {
             hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, st, S.partial.as<double>() + j * 2048,
@@ -253,9 +310,10 @@ struct Out8 { double *p[BK]; };
 
 // out_r = sum_j C[j*ldc + r] X_j  (r < nout <= BK); with Y: out_r = sum_j C[j*ldc+r] (Y_j - w_r X_j)
 // and partial[r * nblk + b] = block sums of out_r^2
-template <int KO>
-__global__ __launch_bounds__(256) void k_block_combo(Ptr64 X, Ptr64 Y, int m, int has_y, const double *__restrict__ C,
-                                                     int ldc, const double *__restrict__ w, Out8 out,
+// (the sums run over j in order; the loads of VU vectors are issued together)
+template <int KO, bool HY>
+__global__ __launch_bounds__(256) void k_block_combo(Ptr64 X, Ptr64 Y, int m, const double *__restrict__ C, int ldc,
+                                                     const double *__restrict__ w, Out8 out,
                                                      double *__restrict__ partial, i64 n) {
     double nrm[KO];
 #pragma unroll
@@ -264,19 +322,29 @@ __global__ __launch_bounds__(256) void k_block_combo(Ptr64 X, Ptr64 Y, int m, in
         double x[KO], y[KO];
 #pragma unroll
         for (int r = 0; r < KO; ++r) x[r] = y[r] = 0.0;
-        for (int j = 0; j < m; ++j) {
-            const double xv = X.p[j][i];
-            const double yv = has_y ? Y.p[j][i] : 0.0;
+        auto step = [&](int j, double xv, double yv) {
 #pragma unroll
             for (int r = 0; r < KO; ++r) {
                 const double c = C[j * ldc + r];
                 x[r] += c * xv;
-                y[r] += c * yv;
+                if (HY) y[r] += c * yv;
             }
+        };
+        int j = 0;
+        for (; j + VU <= m; j += VU) {
+            double xv[VU], yv[VU];
+#pragma unroll
+            for (int u = 0; u < VU; ++u) {
+                xv[u] = X.p[j + u][i];
+                yv[u] = HY ? Y.p[j + u][i] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < VU; ++u) step(j + u, xv[u], yv[u]);
         }
+        for (; j < m; ++j) step(j, X.p[j][i], HY ? Y.p[j][i] : 0.0);
 #pragma unroll
         for (int r = 0; r < KO; ++r) {
-            const double o = has_y ? y[r] - w[r] * x[r] : x[r];
+            const double o = HY ? y[r] - w[r] * x[r] : x[r];
             out.p[r][i] = o;
             nrm[r] += o * o;
         }
@@ -298,7 +366,17 @@ __global__ __launch_bounds__(256) void k_block_subtract(Ptr64 V, int m, const do
         double acc[KO];
 #pragma unroll
         for (int r = 0; r < KO; ++r) acc[r] = 0.0;
-        for (int j = 0; j < m; ++j) {
+        int j = 0;
+        for (; j + VU <= m; j += VU) {
+            double v[VU];
+#pragma unroll
+            for (int u = 0; u < VU; ++u) v[u] = V.p[j + u][i];
+#pragma unroll
+            for (int u = 0; u < VU; ++u)
+#pragma unroll
+                for (int r = 0; r < KO; ++r) acc[r] += P[(j + u) * ldp + r] * v[u];
+        }
+        for (; j < m; ++j) {
             const double v = V.p[j][i];
 #pragma unroll
             for (int r = 0; r < KO; ++r) acc[r] += P[j * ldp + r] * v;
@@ -308,36 +386,34 @@ __global__ __launch_bounds__(256) void k_block_subtract(Ptr64 V, int m, const do
     }
 }
 
-// partial[(a * nb + b) * nblk + blk] = block sums of A_a . B_b   (a < na <= 4, b < nb <= BK)
-template <int NA>
-__global__ __launch_bounds__(256) void k_block_dots(Ptr64 A, int na, Ptr64 B, int nb, double *__restrict__ partial,
-                                                    i64 n) {
-    double acc[NA][BK];
+// partial[(a * NB + b) * nblk + blk] = block sums of A_a . B_b   (a < NA, b < NB, NA * NB <= 32);
+// every pointer is loaded (the host repeats a vector to fill a partial group)
+constexpr int DOT_ACC = 32;
+template <int NA, int NB>
+__global__ __launch_bounds__(256) void k_block_dots(Ptr64 A, Ptr64 B, double *__restrict__ partial, i64 n) {
+    static_assert(NA * NB <= DOT_ACC, "accumulator budget");
+    double acc[NA][NB];
 #pragma unroll
     for (int a = 0; a < NA; ++a)
 #pragma unroll
-        for (int b = 0; b < BK; ++b) acc[a][b] = 0.0;
+        for (int b = 0; b < NB; ++b) acc[a][b] = 0.0;
     for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
-        double bv[BK];
+        double bv[NB], av[NA];
 #pragma unroll
-        for (int b = 0; b < BK; ++b) bv[b] = b < nb ? B.p[b][i] : 0.0;
+        for (int b = 0; b < NB; ++b) bv[b] = B.p[b][i];
 #pragma unroll
-        for (int a = 0; a < NA; ++a) {
-            if (a < na) {
-                const double av = A.p[a][i];
+        for (int a = 0; a < NA; ++a) av[a] = A.p[a][i];
 #pragma unroll
-                for (int b = 0; b < BK; ++b) acc[a][b] += av * bv[b];
-            }
-        }
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int b = 0; b < NB; ++b) acc[a][b] += av[a] * bv[b];
     }
 #pragma unroll
     for (int a = 0; a < NA; ++a)
 #pragma unroll
-        for (int b = 0; b < BK; ++b) {
-            if (a < na && b < nb) {
-                const double t = block_sum_d(acc[a][b]);
-                if (threadIdx.x == 0) partial[(a * nb + b) * gridDim.x + blockIdx.x] = t;
-            }
+        for (int b = 0; b < NB; ++b) {
+            const double t = block_sum_d(acc[a][b]);
+            if (threadIdx.x == 0) partial[(a * NB + b) * gridDim.x + blockIdx.x] = t;
         }
 }
 
@@ -353,7 +429,7 @@ __global__ __launch_bounds__(256) void k_multi_reduce(const double *__restrict__
 struct BlockScratch {
     DevBuf partial, coef, small;
     BlockScratch() {
-        partial.alloc((size_t)4 * BK * 512 * sizeof(double));
+        partial.alloc((size_t)DOT_ACC * 2048 * sizeof(double));
         coef.alloc((size_t)DMAX * DMAX * sizeof(double));
         small.alloc((size_t)4 * DMAX * BK * sizeof(double));
     }
@@ -363,7 +439,7 @@ static BlockScratch &bscratch() {
     if (!s) s.reset(new BlockScratch());
     return *s;
 }
-static int nblk_block(i64 n) { return (int)std::max<i64>(1, std::min<i64>((n + 255) / 256, 512)); }
+static int nblk_block(i64 n) { return (int)std::max<i64>(1, std::min<i64>((n + 255) / 256, 2048)); }
 
 template <int KO>
 static void launch_combo(i64 N, const double *const *X, const double *const *Y, int m, const double *Cdev, int ldc,
@@ -375,23 +451,13 @@ static void launch_combo(i64 N, const double *const *X, const double *const *Y, 
     }
     Out8 O{};
     for (int r = 0; r < KO; ++r) O.p[r] = out[r];
-    hipLaunchKernelGGL(k_block_combo<KO>, dim3(nblk_block(N)), dim3(256), 0, st, PX, PY, m, Y ? 1 : 0, Cdev, ldc,
-                       wdev, O, partial, N);
+    if (Y)
+        hipLaunchKernelGGL((k_block_combo<KO, true>), dim3(nblk_block(N)), dim3(256), 0, st, PX, PY, m, Cdev, ldc,
+                           wdev, O, partial, N);
+    else
+        hipLaunchKernelGGL((k_block_combo<KO, false>), dim3(nblk_block(N)), dim3(256), 0, st, PX, PY, m, Cdev, ldc,
+                           wdev, O, partial, N);
     HIPOTF_CHECK(hipGetLastError());
-}
-
-template <typename F>
-static void dispatch_k(int k, F &&f) {
-    switch (k) {
-    case 1: f(std::integral_constant<int, 1>{}); break;
-    case 2: f(std::integral_constant<int, 2>{}); break;
-    case 3: f(std::integral_constant<int, 3>{}); break;
-    case 4: f(std::integral_constant<int, 4>{}); break;
-    case 5: f(std::integral_constant<int, 5>{}); break;
-    case 6: f(std::integral_constant<int, 6>{}); break;
-    case 7: f(std::integral_constant<int, 7>{}); break;
-    default: f(std::integral_constant<int, 8>{}); break;
-    }
 }
 
 // out_r = sum_j C[j][r] X_j (Y == nullptr) or residuals sum_j C[j][r] (Y_j - w_r X_j) with their
@@ -411,7 +477,7 @@ void block_combine(i64 N, int m, int k, const double *C, const double *w, const 
         const double *Cd = S.coef.as<double>();
         const double *wd = Cd + (size_t)m * BK;
         double *part = norms ? S.partial.as<double>() : nullptr;
-        dispatch_k(kk, [&](auto kc) {
+        dispatch_m8(kk, [&](auto kc) {
             constexpr int KO = decltype(kc)::value;
             launch_combo<KO>(N, X, Y, m, Cd, BK, wd, out + r0, part, st);
         });
@@ -433,21 +499,46 @@ void block_combine(i64 N, int m, int k, const double *C, const double *w, const 
 void block_dots(i64 N, int na, const double *const *A, int nbv, const double *const *B, double *G,
                 hipStream_t st) {
     HIPOTF_REQUIRE(na >= 0 && na <= DMAX && nbv >= 1 && nbv <= BK, "block_dots: bad sizes");
+    if (na == 0) return;
     auto &S = bscratch();
     const int nb = nblk_block(N);
     Ptr64 PB{};
     for (int b = 0; b < nbv; ++b) PB.p[b] = B[b];
-    std::vector<double> h((size_t)4 * BK);
-    for (int a0 = 0; a0 < na; a0 += 4) {
-        const int ka = std::min(4, na - a0);
+    // A vectors per launch: a power of two with NA * nbv <= 32 (at most 16); the groups are
+    // balanced, and a group smaller than its NA repeats its first vector (results dropped)
+    int cap = 16;
+    while (cap * nbv > DOT_ACC) cap /= 2;
+    const int ngroups = (na + cap - 1) / cap, per = (na + ngroups - 1) / ngroups;
+    int NA = 1;
+    while (NA < per) NA *= 2;
+    HIPOTF_REQUIRE(NA * nbv <= DOT_ACC, "block_dots: group too large");
+    std::vector<double> h((size_t)DOT_ACC);
+    for (int a0 = 0; a0 < na; a0 += per) {
+        const int ka = std::min(per, na - a0);
         Ptr64 PA{};
-        for (int a = 0; a < ka; ++a) PA.p[a] = A[a0 + a];
-        hipLaunchKernelGGL(k_block_dots<4>, dim3(nb), dim3(256), 0, st, PA, ka, PB, nbv, S.partial.as<double>(), N);
+        for (int a = 0; a < NA; ++a) PA.p[a] = A[a0 + (a < ka ? a : 0)];
+        double *part = S.partial.as<double>();
+        dispatch_m8(nbv, [&](auto bc) {
+            constexpr int NB = decltype(bc)::value;
+            switch (NA) {
+            case 1: hipLaunchKernelGGL((k_block_dots<1, NB>), dim3(nb), dim3(256), 0, st, PA, PB, part, N); break;
+            case 2: hipLaunchKernelGGL((k_block_dots<2, NB>), dim3(nb), dim3(256), 0, st, PA, PB, part, N); break;
+            case 4: hipLaunchKernelGGL((k_block_dots<4, (NB <= 8 ? NB : 8)>), dim3(nb), dim3(256), 0, st, PA, PB,
+                                       part, N); break;
+            case 8:
+                if constexpr (NB <= 4)
+                    hipLaunchKernelGGL((k_block_dots<8, NB>), dim3(nb), dim3(256), 0, st, PA, PB, part, N);
+                break;
+            default:
+                if constexpr (NB <= 2)
+                    hipLaunchKernelGGL((k_block_dots<16, NB>), dim3(nb), dim3(256), 0, st, PA, PB, part, N);
+                break;
+            }
+        });
         HIPOTF_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(k_multi_reduce, dim3(ka * nbv), dim3(256), 0, st, S.partial.as<double>(), nb,
-                           S.small.as<double>());
+        hipLaunchKernelGGL(k_multi_reduce, dim3(NA * nbv), dim3(256), 0, st, part, nb, S.small.as<double>());
         HIPOTF_CHECK(hipGetLastError());
-        HIPOTF_CHECK(hipMemcpyAsync(h.data(), S.small.p, (size_t)ka * nbv * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPOTF_CHECK(hipMemcpyAsync(h.data(), S.small.p, (size_t)NA * nbv * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPOTF_CHECK(hipStreamSynchronize(st));
         for (int a = 0; a < ka; ++a)
             for (int b = 0; b < nbv; ++b) G[(size_t)(a0 + a) * nbv + b] = h[(size_t)a * nbv + b];
@@ -468,7 +559,7 @@ void block_subtract(i64 N, int m, const double *const *V, const double *P, int n
     for (int i = 0; i < m; ++i) PV.p[i] = V[i];
     Out8 O{};
     for (int j = 0; j < nt; ++j) O.p[j] = T[j];
-    dispatch_k(nt, [&](auto kc) {
+    dispatch_m8(nt, [&](auto kc) {
         constexpr int KO = decltype(kc)::value;
         hipLaunchKernelGGL(k_block_subtract<KO>, dim3(nblk_block(N)), dim3(256), 0, st, PV, m,
                            S.coef.as<double>(), BK, O, N);
