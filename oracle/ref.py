@@ -1,8 +1,9 @@
 """ORACLE (test infrastructure only) -- ctypes bindings to the reference's own CPU libraries.
 
 ``oracle/Makefile`` compiles them from the reference sources into ``oracle/_ref``; this
-module only loads what is there.  Used by tests/ (to pin the restatement) and by
-``tests/golden/make_golden.py``; never by the product.
+module only loads what is there.  Used by tests/ (to pin the restatement), by
+``tests/golden/make_golden.py`` and by the CPU-baseline leg of ``bench.py``; never by the
+product.
 
 Signatures follow src/include/op/declare.hpp:13-132, src/include/diag/declare.hpp:4-75,
 src/include/basis/declare.hpp:4-81 and src/include/ham/declare.hpp:7-27.
@@ -152,3 +153,74 @@ class Reference:
         for i, v in enumerate(vecs):
             arr[i] = _ptr(v, _dp)
         return arr
+
+    def davidson(self, aop, v0, diags, tol, maxiter=100000):
+        """jl/davidson.jl:1-185 (one root, ncv = 3, maxspace = ncv + 20) driving this library's
+        libdiag ops.  Returns (energy, iterations, converged); stops after ``maxiter``."""
+        d = self.diag
+        N = len(v0)
+        ncv, maxspace = 3, 23
+        P = lambda a: _ptr(a, _dp)  # noqa: E731
+        V, AV = [np.array(v0, dtype=np.float64)], [np.empty(N)]
+        d.inplace_normalize_f64(N, P(V[0]))
+        AV[0][:] = aop(V[0])
+        heff = np.zeros((maxspace, maxspace))
+        heff[0, 0] = d.fast_dot_f64(N, P(V[0]), P(AV[0]))
+        axt = np.empty(N)
+        dim, e = 1, float(heff[0, 0])
+        for it in range(1, maxiter + 1):
+            es, vs = np.linalg.eigh(heff[:dim, :dim])
+            k = int(np.argmin(es))
+            e = float(es[k])
+            c = np.ascontiguousarray(vs[:, k])
+            nr = d.build_ritz_and_residual_f64(N, dim, P(c), e, self.ptr_array(V[:dim]),
+                                               self.ptr_array(AV[:dim]), P(axt))
+            if nr < tol:
+                return e, it, True
+            if dim >= maxspace:  # restart to the ncv lowest Ritz vectors (davidson.jl:82-114)
+                order = np.argsort(es)
+                Vn = [sum(vs[j, order[i]] * V[j] for j in range(dim)) for i in range(ncv)]
+                AVn = [sum(vs[j, order[i]] * AV[j] for j in range(dim)) for i in range(ncv)]
+                for i in range(ncv):
+                    V[i][:], AV[i][:] = Vn[i], AVn[i]
+                dim = ncv
+                heff_c = np.zeros(maxspace * maxspace)
+                d.rebuild_heff_f64(N, dim, self.ptr_array(V[:dim]), self.ptr_array(AV[:dim]),
+                                   P(heff_c), maxspace)
+                heff[:, :] = heff_c.reshape(maxspace, maxspace).T
+            shift = max(nr * 1e-2, 1e-4)
+            d.apply_preconditioner_inplace_f64(N, P(axt), P(diags), e, shift)
+            nb_ = float(np.linalg.norm(axt))
+            na_ = d.mgs_orthogonalize_f64(N, dim, self.ptr_array(V[:dim]), P(axt))
+            if na_ < 0.717 * nb_:
+                na_ = d.mgs_orthogonalize_f64(N, dim, self.ptr_array(V[:dim]), P(axt))
+            if na_ < 1e-12:
+                return e, it, True
+            d.inplace_normalize_f64(N, P(axt))
+            dim += 1
+            if len(V) < dim:
+                V.append(np.empty(N))
+                AV.append(np.empty(N))
+            V[dim - 1][:] = axt
+            AV[dim - 1][:] = aop(V[dim - 1])
+            col = np.zeros(dim)
+            d.compute_heff_col_f64(N, dim, self.ptr_array(V[:dim]), P(AV[dim - 1]), P(col))
+            heff[:dim, dim - 1] = col
+            heff[dim - 1, :dim] = col
+        return e, maxiter, False
+
+    def energy_objective(self, basis, ham, pool, x, dim):
+        """jl/vqe.jl:1-21 from the Hartree-Fock state (index 0) with 0-based pool indices
+        0..len(x)-1, on this library's expm / hvec / backgrad.  Returns (energy, grad, dH2)."""
+        lv = np.zeros(dim)
+        lv[0] = 1.0
+        for i, t in enumerate(x):
+            self.expm(basis, pool, i, float(t), lv)
+        rv = self.hvec(basis, ham, lv)
+        lnorm, rnorm = float(lv @ lv), float(rv @ rv)
+        e = float(lv @ rv) / lnorm
+        d2 = max(0.0, rnorm / lnorm - e * e)
+        g = np.zeros(len(x))
+        for i in range(len(x) - 1, -1, -1):
+            g[i] = self.backgrad(basis, pool, i, float(x[i]), lv, rv) * 2.0 / lnorm
+        return e, g, d2

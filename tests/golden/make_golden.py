@@ -76,61 +76,6 @@ def small_fixtures(R):
         print("small", n, na, nb, "dim", dim, "pool", len(sos), flush=True)
 
 
-def davidson_ref(R, aop, v0, diags, tol):
-    """jl/davidson.jl:1-185 driving the reference's libdiag (oracle.oracle.davidson mirrors it)."""
-    import ctypes as C
-    dp = C.POINTER(C.c_double)
-    d = R.diag
-    N = len(v0)
-    ncv, maxspace = 3, 23
-    P = lambda a: a.ctypes.data_as(dp)  # noqa: E731
-    V, AV = [v0.copy()], [np.empty(N)]
-    d.inplace_normalize_f64(N, P(V[0]))
-    AV[0][:] = aop(V[0])
-    heff = np.zeros((maxspace, maxspace))
-    heff[0, 0] = d.fast_dot_f64(N, P(V[0]), P(AV[0]))
-    axt = np.empty(N)
-    dim = 1
-    for it in range(1, 100000):
-        es, vs = np.linalg.eigh(heff[:dim, :dim])
-        k = int(np.argmin(es))
-        e = float(es[k])
-        c = np.ascontiguousarray(vs[:, k])
-        nr = d.build_ritz_and_residual_f64(N, dim, P(c), e, R.ptr_array(V[:dim]),
-                                           R.ptr_array(AV[:dim]), P(axt))
-        if nr < tol:
-            return e, it
-        if dim >= maxspace:
-            order = np.argsort(es)
-            Vn = [sum(vs[j, order[i]] * V[j] for j in range(dim)) for i in range(ncv)]
-            AVn = [sum(vs[j, order[i]] * AV[j] for j in range(dim)) for i in range(ncv)]
-            for i in range(ncv):
-                V[i][:], AV[i][:] = Vn[i], AVn[i]
-            dim = ncv
-            d.rebuild_heff_f64(N, dim, R.ptr_array(V[:dim]), R.ptr_array(AV[:dim]), P(heff_c := np.zeros(maxspace * maxspace)), maxspace)
-            heff[:, :] = heff_c.reshape(maxspace, maxspace).T
-        shift = max(nr * 1e-2, 1e-4)
-        d.apply_preconditioner_inplace_f64(N, P(axt), P(diags), e, shift)
-        nb_ = float(np.linalg.norm(axt))
-        na_ = d.mgs_orthogonalize_f64(N, dim, R.ptr_array(V[:dim]), P(axt))
-        if na_ < 0.717 * nb_:
-            na_ = d.mgs_orthogonalize_f64(N, dim, R.ptr_array(V[:dim]), P(axt))
-        if na_ < 1e-12:
-            return e, it
-        d.inplace_normalize_f64(N, P(axt))
-        dim += 1
-        if len(V) < dim:
-            V.append(np.empty(N))
-            AV.append(np.empty(N))
-        V[dim - 1][:] = axt
-        AV[dim - 1][:] = aop(V[dim - 1])
-        col = np.zeros(dim)
-        d.compute_heff_col_f64(N, dim, R.ptr_array(V[:dim]), P(AV[dim - 1]), P(col))
-        heff[:dim, dim - 1] = col
-        heff[dim - 1, :dim] = col
-    raise RuntimeError("no convergence")
-
-
 def config0_fixtures(R):
     n, na, nb = 12, 6, 6
     h, eri = for_norb(n)
@@ -149,22 +94,13 @@ def config0_fixtures(R):
     hf = np.zeros(dim)
     hf[0] = 1.0
     t2 = time.time()
-    e0, iters = davidson_ref(R, lambda v: R.hvec(S["basis"], S["ham"], v), hf, diag, 1e-8)
+    e0, iters, _ = R.davidson(lambda v: R.hvec(S["basis"], S["ham"], v), hf, diag, 1e-8)
     t_dav = time.time() - t2
     # VQE energy + gradient (config 2: all 1818 parameters, theta ~ U[-0.05, 0.05], seed+2000)
     sos = S["sos"]
     th = np.random.default_rng(20261202 + 2000).uniform(-0.05, 0.05, len(sos))
-    lv = hf.copy()
     t3 = time.time()
-    for i, t in enumerate(th):
-        R.expm(S["basis"], S["pool"], i, t, lv)
-    rv = R.hvec(S["basis"], S["ham"], lv)
-    lnorm, rnorm = float(lv @ lv), float(rv @ rv)
-    e = float(lv @ rv) / lnorm
-    d2 = max(0.0, rnorm / lnorm - e * e)
-    g = np.zeros(len(sos))
-    for i in range(len(sos) - 1, -1, -1):
-        g[i] = R.backgrad(S["basis"], S["pool"], i, th[i], lv, rv) * 2.0 / lnorm
+    e, g, d2 = R.energy_objective(S["basis"], S["ham"], S["pool"], th, dim)
     t_vqe = time.time() - t3
     np.savez_compressed(os.path.join(HERE, "config0_n12.npz"), sigma_sum=sigma.sum(),
                         sigma_norm2=sigma @ sigma, sigma_dot_y=y @ sigma, sample_idx=samp,
