@@ -251,8 +251,9 @@ def main():
                          "of applying the stored choice")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on exactly one GPU: there is no multi-GPU code path (the sigma "
+              "would shard by destination alpha rows after an all-gather of x; not built)",
+              file=sys.stderr)
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), "no GPU visible"
