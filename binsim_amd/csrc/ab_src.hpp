@@ -23,6 +23,10 @@
 // into the LDS row one wave at a time, in wave order, so the sum order -- and the FP result
 // -- is reproducible; otherwise all waves add at once.  Fixed block size, so fill positions
 // are compile-time offsets.
+//
+// The LDS row holds the destination elements [lo, lo + nbp) of row Ia; contributions to other
+// elements are dropped.  A row too long for the LDS is done in parts, one launch per part
+// (every part repeats the accumulation and keeps its own targets).
 // Included inside namespace hipotf after sx_* and u32/u64/i64 are defined.
 #pragma once
 
@@ -42,7 +46,7 @@ __global__ __launch_bounds__(AB_SRC_NJ *SPLIT) __attribute__((amdgpu_waves_per_e
 void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, const u32 *__restrict__ sxA,
               const u32 *__restrict__ sxB, const u64 *__restrict__ strsA, const u64 *__restrict__ strsB,
               const T *__restrict__ eri, const T *__restrict__ dw, int n, i64 NA, i64 NB, int nsxA, int kB,
-              int vB, int ordered) {
+              int vB, int ordered, i64 lo, i64 nbp) {
     constexpr int BD = AB_SRC_NJ * SPLIT;
     constexpr int KH = KM / SPLIT;                 // occupied-orbital slots per thread
     constexpr int XPF = AB_SRC_XPF;
@@ -65,9 +69,9 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
     // (one barrier per connection instead of two)
     T *Wr = reinterpret_cast<T *>(smem_raw);       // DB x NW * R replicated integral rows
     T *Z = Wr + DB * NW * R;                       // 16 zeros (reads of unused slots)
-    T *S = Z + 16;                                 // NB * NVS destination row
+    T *S = Z + 16;                                 // nbp * NVS destination elements lo..
     constexpr int BW = KM + 2;
-    int *binom = reinterpret_cast<int *>(S + NB * NVS);  // C(p, j), p < n, j < BW
+    int *binom = reinterpret_cast<int *>(S + nbp * NVS);  // C(p, j), p < n, j < BW
     // the row's alpha connections (single-excitation table entries of Ia), staged so the round
     // loop issues no scalar memory loads: those share the LDS wait counter and would force
     // every wait on the slot gathers to drain all of them
@@ -79,7 +83,7 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
     const i64 Ia = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (Ia >= NA) return;
     (void)strsA;
-    for (i64 t = tid; t < NB * NVS; t += BD) S[t] = T(0);
+    for (i64 t = tid; t < nbp * NVS; t += BD) S[t] = T(0);
     if (tid < 16) Z[tid] = T(0);
     for (int t = tid; t < n * BW; t += BD) {
         const int p = t / BW, j = t - p * BW;
@@ -334,25 +338,29 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
                                 const int io = half * KH + i;
                                 if (io < kB) {
                                     const bool above = m > io;
-                                    const i64 Ib = Jb + (above ? up[i] + F : dn[i] + G);
+                                    const i64 Ip = Jb + (above ? up[i] + F : dn[i] + G) - lo;
                                     const bool neg = (m + io + (above ? 1 : 0)) & 1;
+                                    if ((u64)Ip < (u64)nbp) {
 #pragma unroll
-                                    for (int v = 0; v < NVS; ++v)
-                                        atomicAdd(&S[Ib * NVS + v], neg ? -acc[i][iv][v] : acc[i][iv][v]);
+                                        for (int v = 0; v < NVS; ++v)
+                                            atomicAdd(&S[Ip * NVS + v], neg ? -acc[i][iv][v] : acc[i][iv][v]);
+                                    }
                                 }
                             }
                         }
                     }
+                    if ((u64)(Jb - lo) < (u64)nbp) {
 #pragma unroll
-                    for (int v = 0; v < NVS; ++v) atomicAdd(&S[Jb * NVS + v], accd[v]);
+                        for (int v = 0; v < NVS; ++v) atomicAdd(&S[(Jb - lo) * NVS + v], accd[v]);
+                    }
                 }
                 __syncthreads();
             }
         }
     }
     __syncthreads();
-    for (i64 t = tid; t < NB * NVS; t += BD) {
-        const i64 Ib = t / NVS;
-        s1[(Ia * NB + Ib) * nvst + v0 + (t - Ib * NVS)] += S[t];
+    for (i64 t = tid; t < nbp * NVS; t += BD) {
+        const i64 Ip = t / NVS;
+        s1[(Ia * NB + lo + Ip) * nvst + v0 + (t - Ip * NVS)] += S[t];
     }
 }

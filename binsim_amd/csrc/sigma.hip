@@ -660,10 +660,11 @@ static bool try_ab_src(DevHam *H, const T *xt, T *s1, int nvst, int v0, const T 
         // copies make them conflict-free; 8-byte reads get 16 (LDS space).  (FP32 with 16
         // copies measured the same at 16 orbitals.)
         if (H->n * H->n <= 256) return try_ab_src_r<T, NVS, sizeof(T) == 8 ? 16 : 32, 256>(H, xt, s1, nvst, v0, eri, st);
-        // 17-22 orbitals: 512-entry rows; only the single-vector FP32 kernel fits (the FP32
-        // destination row of 18 orbitals / 7 electrons is 127 KB of LDS, 16 copies of the row
-        // 32 KB; in FP64 the row alone exceeds the LDS and 77 accumulators the registers)
-        if constexpr (sizeof(T) == 4 && NVS == 1) return try_ab_src_r<T, NVS, 16, 512>(H, xt, s1, nvst, v0, eri, st);
+        // 17-22 orbitals: 512-entry rows, single-vector kernels only.  FP32: the destination row
+        // of 18 orbitals / 7 electrons is 127 KB of LDS, 16 copies of the integral row 32 KB.
+        // FP64: the 254 KB row is done in two halves (8 copies of the integral row, 32 KB) and
+        // at most 7 x 11 slots per string (77 accumulators)
+        if constexpr (NVS == 1) return try_ab_src_r<T, NVS, sizeof(T) == 8 ? 8 : 16, 512>(H, xt, s1, nvst, v0, eri, st);
         return false;
     }
 }
@@ -686,13 +687,18 @@ static bool try_ab_src_r(DevHam *H, const T *xt, T *s1, int nvst, int v0, const 
         HIPOTF_REQUIRE(NW - n2 <= ERI_PAD, "internal: integral table padding too short for the row width");
         // + the binomial table C(p, j), p < n <= 18, j < KM + 2 <= 10; two replicated rows
         // (one barrier per connection) if tuned so and they fit
-        auto lds_for = [&](int db) {
-            return ((size_t)db * NW * R + 16 + (size_t)NB * NVS) * sizeof(T) +
+        auto lds_for = [&](int db, i64 nbp) {
+            return ((size_t)db * NW * R + 16 + (size_t)nbp * NVS) * sizeof(T) +
                    (AB_SRC_BINOM + AB_SRC_CONN) * 4;
         };
         const size_t lds_max = std::min<size_t>(lds_budget(), 160 * 1024);
-        const int db = (tuning(prec).ab_db == 2 && lds_for(2) <= lds_max) ? 2 : 1;
-        const size_t smem = lds_for(db);
+        const int db = (tuning(prec).ab_db == 2 && lds_for(2, NB) <= lds_max) ? 2 : 1;
+        // destination-row parts: one, or (FP64 512-entry form) as few as fit the LDS
+        int parts = 1;
+        if (NW == 512 && sizeof(T) == 8)
+            while (parts < 4 && lds_for(db, (NB + parts - 1) / parts) > lds_max) ++parts;
+        const i64 nbp = (NB + parts - 1) / parts;
+        const size_t smem = lds_for(db, nbp);
         if (smem > lds_max) return false;
         const i64 per = (A.N + 7) / 8;
         const i64 nblocks = per * 8;
@@ -701,14 +707,23 @@ static bool try_ab_src_r(DevHam *H, const T *xt, T *s1, int nvst, int v0, const 
         auto go = [&](auto kern) {
             HIPOTF_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                              (int)smem));
-            hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(bd), smem, st, xt, s1, nvst, v0,
-                               A.sx.as<u32>(), Bs.sx.as<u32>(), A.strs.as<u64>(), Bs.strs.as<u64>(), eri,
-                               dw, n, A.N, NB, A.nsx, kB, vB, deterministic(prec) ? 1 : 0);
-            HIPOTF_CHECK(hipGetLastError());
+            for (int part = 0; part < parts; ++part) {
+                const i64 lo = part * nbp, cnt = std::min<i64>(nbp, NB - lo);
+                hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(bd), smem, st, xt, s1, nvst, v0,
+                                   A.sx.as<u32>(), Bs.sx.as<u32>(), A.strs.as<u64>(), Bs.strs.as<u64>(), eri,
+                                   dw, n, A.N, NB, A.nsx, kB, vB, deterministic(prec) ? 1 : 0, lo, cnt);
+                HIPOTF_CHECK(hipGetLastError());
+            }
         };
         if constexpr (NW == 512) {
-            // 17-18 orbitals: up to 8 occupied and 12 virtual orbitals per beta string
-            if (kB <= 8 && vB <= 12 && db == 1) {
+            // 17-18 orbitals: up to 8 occupied and 12 virtual orbitals per beta string (FP32),
+            // 7 and 11 (FP64)
+            if constexpr (sizeof(T) == 8) {
+                if (kB <= 7 && vB <= 11 && db == 1) {
+                    go(k_ab_src<T, NVS, 7, 11, R, 1, 1, NW>);
+                    return true;
+                }
+            } else if (kB <= 8 && vB <= 12 && db == 1) {
                 go(k_ab_src<T, NVS, 8, 12, R, 1, 1, NW>);
                 return true;
             }

@@ -21,9 +21,10 @@ def _dev(n, na, nb, seed=None):
 
 
 # beyond 16 orbitals (36-bit interleaved determinants).  The alpha-beta part runs on the
-# fallback kernels, except in FP32 with at most 12 beta holes ((18, 1, 6), (17, 2, 5)), where the
-# source-lane kernel's 512-entry-row form runs
-WIDE = [(17, 3, 3), (18, 2, 2), (18, 3, 2), (18, 1, 6), (17, 2, 5)]
+# fallback kernels, except where the source-lane kernel's 512-entry-row form takes it: FP32 with
+# at most 8 beta electrons and 12 holes ((18, 1, 6), (17, 2, 5)), FP64 with at most 7 and 11
+# ((17, 1, 6) in one launch; (18, 1, 7) with its 254 KB destination row in two halves)
+WIDE = [(17, 3, 3), (18, 2, 2), (18, 3, 2), (18, 1, 6), (17, 2, 5), (17, 1, 6), (18, 1, 7)]
 
 
 @pytest.mark.parametrize("n,na,nb", SMALL + [(12, 6, 6)] + WIDE)
