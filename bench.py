@@ -19,6 +19,9 @@ Reported beside it (one run each, outside the timed steps):
     bounded sample, all host cores.
 
     python bench.py [--steps K] [--warmup W] [--quick]
+    python bench.py --davidson-only 600     # 16-orbital ground state, plain vs mixed
+    python bench.py --sigma-shape 18,7      # FP64 / FP32 sigma at another shape
+    python bench.py --vqe-only 16           # VQE energy + gradient only
 """
 from __future__ import annotations
 
@@ -101,6 +104,32 @@ def fci_block(torch, B, H, nroots, maxspace, tol, maxiter):
     de = np.abs(np.array(out["plain_fp64"]["energies"]) - np.array(out["mixed_block"]["energies"]))
     out["abs_dE_per_root"] = [float(v) for v in de]
     out["speedup_mixed_vs_plain"] = round(out["plain_fp64"]["seconds"] / out["mixed_block"]["seconds"], 3)
+    return out
+
+
+def sigma_shape(torch, n, k, steps, warmup):
+    """FP64 and single-vector FP32 sigma for n orbitals, k+k electrons, on a seeded vector,
+    wall clock around ``steps`` calls after ``warmup`` (stored launch choices where present)."""
+    from binsim_amd import autotune as AT
+    B, H, _, F = build(n, k, k)
+    entry = AT.lookup(B)
+    if entry is not None:
+        AT.apply(entry)
+    x = seeded(torch, B.dim, 1000 + n)
+    y, y32 = torch.empty_like(x), torch.empty_like(x)
+    out = {"sigma_shape": f"{n} orbitals {k}a{k}b", "dim": B.dim, "steps": steps}
+    for name, fn, w in (("fp64", lambda: F.hvec(x, y), 8), ("fp32", lambda: F.hvec_block([x], [y32], 1), 4)):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / steps
+        out[name] = dict(seconds=round(dt, 4), algorithmic_GBps=round(sigma_bytes(n, k) * w / 8 / dt / 1e9, 1))
+        log(f"sigma {name} n={n} k={k}: {dt * 1e3:.1f} ms")
+    out["fp32_rel_dev"] = float((y32 - y).norm() / y.norm())
     return out
 
 
@@ -246,6 +275,8 @@ def main():
     ap.add_argument("--davidson-only", type=int, default=None, metavar="MAXITER",
                     help="only the 16-orbital ground state to 1e-6: plain FP64, then mixed (block 1, "
                          "maxspace 32), each stopped after MAXITER iterations (600: to convergence)")
+    ap.add_argument("--sigma-shape", default=None, metavar="N,K",
+                    help="only time FP64 and single-vector FP32 sigma for N orbitals, K+K electrons")
     ap.add_argument("--tune", action="store_true",
                     help="run the sigma autotuner on the workload first (about a minute) instead "
                          "of applying the stored choice")
@@ -265,6 +296,10 @@ def main():
         return
     if args.davidson_only:
         print(json.dumps(fci16_ground(torch, args.davidson_only)))
+        return
+    if args.sigma_shape:
+        nn, kk = (int(v) for v in args.sigma_shape.split(","))
+        print(json.dumps(sigma_shape(torch, nn, kk, args.steps, args.warmup)))
         return
     n, na, nb = 16, 8, 8
     t = time.perf_counter()

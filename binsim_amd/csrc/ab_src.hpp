@@ -46,10 +46,14 @@ __global__ __launch_bounds__(AB_SRC_NJ *SPLIT) __attribute__((amdgpu_waves_per_e
 void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, const u32 *__restrict__ sxA,
               const u32 *__restrict__ sxB, const u64 *__restrict__ strsA, const u64 *__restrict__ strsB,
               const T *__restrict__ eri, const T *__restrict__ dw, int n, i64 NA, i64 NB, int nsxA, int kB,
-              int vB, int ordered, i64 lo, i64 nbp) {
+              int vB, int ordered, i64 lo_arg, i64 nbp_arg) {
     constexpr int BD = AB_SRC_NJ * SPLIT;
     constexpr int KH = KM / SPLIT;                 // occupied-orbital slots per thread
     constexpr int XPF = AB_SRC_XPF;
+    // only the FP64 512-entry form is launched in parts; every other form gets lo = 0, nbp = NB
+    // and compiles the range checks of the scatter out
+    constexpr bool PARTS = NW == 512 && sizeof(T) == 8;
+    const i64 lo = PARTS ? lo_arg : 0, nbp = PARTS ? nbp_arg : NB;
     // W fill: wave w loads entries [w*EPW, (w+1)*EPW) of the row, LPE lanes per entry (one
     // coalesced load per thread), and each lane stores BPL 16-byte blocks of CB copies of its
     // entry.  The block order is rotated with the entry so that the 64 lanes of one store
@@ -340,7 +344,7 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
                                     const bool above = m > io;
                                     const i64 Ip = Jb + (above ? up[i] + F : dn[i] + G) - lo;
                                     const bool neg = (m + io + (above ? 1 : 0)) & 1;
-                                    if ((u64)Ip < (u64)nbp) {
+                                    if (!PARTS || (u64)Ip < (u64)nbp) {
 #pragma unroll
                                         for (int v = 0; v < NVS; ++v)
                                             atomicAdd(&S[Ip * NVS + v], neg ? -acc[i][iv][v] : acc[i][iv][v]);
@@ -349,7 +353,7 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
                             }
                         }
                     }
-                    if ((u64)(Jb - lo) < (u64)nbp) {
+                    if (!PARTS || (u64)(Jb - lo) < (u64)nbp) {
 #pragma unroll
                         for (int v = 0; v < NVS; ++v) atomicAdd(&S[(Jb - lo) * NVS + v], accd[v]);
                     }

@@ -658,13 +658,13 @@ static bool try_ab_src(DevHam *H, const T *xt, T *s1, int nvst, int v0, const T 
     } else {
         // R copies of the integral row: 32 lanes of 4-byte reads are served together, so 32
         // copies make them conflict-free; 8-byte reads get 16 (LDS space).  (FP32 with 16
-        // copies measured the same at 16 orbitals.)
+        // copies at 16 orbitals: the same for two vectors, 9% slower for one.)
         if (H->n * H->n <= 256) return try_ab_src_r<T, NVS, sizeof(T) == 8 ? 16 : 32, 256>(H, xt, s1, nvst, v0, eri, st);
-        // 17-22 orbitals: 512-entry rows, single-vector kernels only.  FP32: the destination row
-        // of 18 orbitals / 7 electrons is 127 KB of LDS, 16 copies of the integral row 32 KB.
-        // FP64: the 254 KB row is done in two halves (8 copies of the integral row, 32 KB) and
-        // at most 7 x 11 slots per string (77 accumulators)
-        if constexpr (NVS == 1) return try_ab_src_r<T, NVS, sizeof(T) == 8 ? 8 : 16, 512>(H, xt, s1, nvst, v0, eri, st);
+        // 17-22 orbitals: 512-entry rows, single-vector kernels only, 8 copies of the integral
+        // row (FP32 at 18 orbitals / 7 electrons: 8 copies 4% faster than 16).  FP32: the 127 KB
+        // destination row stays whole.  FP64: the 254 KB row is done in two halves, and at most
+        // 7 x 11 slots per string (77 accumulators)
+        if constexpr (NVS == 1) return try_ab_src_r<T, NVS, 8, 512>(H, xt, s1, nvst, v0, eri, st);
         return false;
     }
 }
@@ -694,7 +694,7 @@ static bool try_ab_src_r(DevHam *H, const T *xt, T *s1, int nvst, int v0, const 
         const size_t lds_max = std::min<size_t>(lds_budget(), 160 * 1024);
         const int db = (tuning(prec).ab_db == 2 && lds_for(2, NB) <= lds_max) ? 2 : 1;
         // destination-row parts: one, or (FP64 512-entry form) as few as fit the LDS
-        int parts = 1;
+        int parts = 1;  // (the kernel checks scatter targets against its part in this form only)
         if (NW == 512 && sizeof(T) == 8)
             while (parts < 4 && lds_for(db, (NB + parts - 1) / parts) > lds_max) ++parts;
         const i64 nbp = (NB + parts - 1) / parts;
@@ -723,6 +723,9 @@ static bool try_ab_src_r(DevHam *H, const T *xt, T *s1, int nvst, int v0, const 
                     go(k_ab_src<T, NVS, 7, 11, R, 1, 1, NW>);
                     return true;
                 }
+            } else if (kB <= 7 && vB <= 11 && db == 1) {
+                go(k_ab_src<T, NVS, 7, 11, R, 1, 1, NW>);  // 18 orbitals / 7 electrons: no empty slots
+                return true;
             } else if (kB <= 8 && vB <= 12 && db == 1) {
                 go(k_ab_src<T, NVS, 8, 12, R, 1, 1, NW>);
                 return true;

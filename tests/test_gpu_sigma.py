@@ -69,6 +69,24 @@ def test_block_sigma(torch_cuda, n, na, nb, k):
         assert rel_err(host(y), r) < 2e-5
 
 
+@pytest.mark.parametrize("n,na,nb", [(17, 1, 6), (18, 1, 7)])
+def test_sigma_512_entry_forms_both_precisions(torch_cuda, n, na, nb):
+    """The source-lane kernel's 512-entry forms with at most 7 beta electrons and 11 holes:
+    FP64 (one launch at (17, 1, 6); the destination row in two halves at (18, 1, 7)) and the
+    single-vector FP32 form, against the oracle."""
+    print(f"512-entry forms n={n} na={na} nb={nb}", flush=True)
+    S = oracle_system(n, na, nb)
+    x = seeded_vector(S.dim, 300 + n)
+    ref = S.hvec(x)
+    B, H, F = _dev(n, na, nb)
+    xd = gpu_vec(torch_cuda, x)
+    yd = torch_cuda.empty_like(xd)
+    F.hvec_block([xd], [yd], precision=0)
+    assert rel_err(host(yd), ref) < 1e-12
+    F.hvec_block([xd], [yd], precision=1)
+    assert rel_err(host(yd), ref) < 2e-5
+
+
 @pytest.mark.parametrize("n,na,nb", [(4, 2, 2), (6, 3, 2), (8, 4, 4)])
 def test_network_handle_decodes_to_same_operator(torch_cuda, n, na, nb):
     """build_network_otf_f64 + build_networkdev_f64 (the drop-in path) == integral path."""
