@@ -40,7 +40,7 @@ PEAK_HBM_GBS = 8000.0
 # `bench.py --pmc-unit` (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, summed over the
 # dispatches of one sigma), each scaled by what the same counter reads for a plain copy kernel
 # of 1.325 GB in the same run (profiles/sigma_fp64_16o_pmc.md)
-SIGMA_TRAFFIC_16O = 268.8e9
+SIGMA_TRAFFIC_16O = 266.4e9
 
 
 def log(*a):
