@@ -83,6 +83,18 @@ def test_sigma_512_entry_forms_both_precisions(torch_cuda, n, na, nb):
     yd = torch_cuda.empty_like(xd)
     F.hvec_block([xd], [yd], precision=0)
     assert rel_err(host(yd), ref) < 1e-12
+    # the ordered scatter is bitwise reproducible, also when the row is done in parts
+    from binsim_amd import autotune as AT
+    saved = AT.get_config(AT.FP64)
+    try:
+        AT.set_config(AT.FP64, dict(saved, deterministic=1))
+        y2 = torch_cuda.empty_like(xd)
+        F.hvec(xd, yd)
+        F.hvec(xd, y2)
+        assert torch_cuda.equal(yd, y2)
+        assert rel_err(host(y2), ref) < 1e-12
+    finally:
+        AT.set_config(AT.FP64, saved)
     F.hvec_block([xd], [yd], precision=1)
     assert rel_err(host(yd), ref) < 2e-5
 
