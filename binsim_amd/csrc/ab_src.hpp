@@ -41,6 +41,12 @@ constexpr int AB_SRC_PD = 1;
 // keep slot addresses in VGPRs across rounds (NVS = 1, DB = 1 only): 1.5-3% faster there
 constexpr bool AB_SRC_HOIST = true;
 
+// word type of the LDS destination row: FP64 also for the single-vector FP32 kernel of up to 16
+// orbitals (its 103 KB row fits beside the integral copies), whose scatter then uses the FP64
+// LDS atomic add
+template <typename T, int NVS, int NW>
+using AbSrcRow = typename std::conditional<NW == 256 && NVS == 1, double, T>::type;
+
 template <typename T, int NVS, int KM, int VM, int R, int SPLIT, int DB, int NW>
 __global__ __launch_bounds__(AB_SRC_NJ *SPLIT) __attribute__((amdgpu_waves_per_eu(2 * SPLIT, 2 * SPLIT)))
 void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, const u32 *__restrict__ sxA,
@@ -73,7 +79,8 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
     // (one barrier per connection instead of two)
     T *Wr = reinterpret_cast<T *>(smem_raw);       // DB x NW * R replicated integral rows
     T *Z = Wr + DB * NW * R;                       // 16 zeros (reads of unused slots)
-    T *S = Z + 16;                                 // nbp * NVS destination elements lo..
+    using ST = AbSrcRow<T, NVS, NW>;
+    ST *S = reinterpret_cast<ST *>(Z + 16);        // nbp * NVS destination elements lo..
     constexpr int BW = KM + 2;
     int *binom = reinterpret_cast<int *>(S + nbp * NVS);  // C(p, j), p < n, j < BW
     // the row's alpha connections (single-excitation table entries of Ia), staged so the round
@@ -87,7 +94,7 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
     const i64 Ia = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (Ia >= NA) return;
     (void)strsA;
-    for (i64 t = tid; t < nbp * NVS; t += BD) S[t] = T(0);
+    for (i64 t = tid; t < nbp * NVS; t += BD) S[t] = ST(0);
     if (tid < 16) Z[tid] = T(0);
     for (int t = tid; t < n * BW; t += BD) {
         const int p = t / BW, j = t - p * BW;
@@ -347,7 +354,7 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
                                     if (!PARTS || (u64)Ip < (u64)nbp) {
 #pragma unroll
                                         for (int v = 0; v < NVS; ++v)
-                                            atomicAdd(&S[Ip * NVS + v], neg ? -acc[i][iv][v] : acc[i][iv][v]);
+                                            atomicAdd(&S[Ip * NVS + v], (ST)(neg ? -acc[i][iv][v] : acc[i][iv][v]));
                                     }
                                 }
                             }
@@ -355,7 +362,7 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
                     }
                     if (!PARTS || (u64)(Jb - lo) < (u64)nbp) {
 #pragma unroll
-                        for (int v = 0; v < NVS; ++v) atomicAdd(&S[(Jb - lo) * NVS + v], accd[v]);
+                        for (int v = 0; v < NVS; ++v) atomicAdd(&S[(Jb - lo) * NVS + v], (ST)accd[v]);
                     }
                 }
                 __syncthreads();
@@ -365,6 +372,7 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
     __syncthreads();
     for (i64 t = tid; t < nbp * NVS; t += BD) {
         const i64 Ip = t / NVS;
-        s1[(Ia * NB + lo + Ip) * nvst + v0 + (t - Ip * NVS)] += S[t];
+        T &o = s1[(Ia * NB + lo + Ip) * nvst + v0 + (t - Ip * NVS)];
+        o = (T)((ST)o + S[t]);
     }
 }

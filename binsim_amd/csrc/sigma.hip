@@ -689,7 +689,7 @@ static bool try_ab_src_r(DevHam *H, const T *xt, T *s1, int nvst, int v0, const 
         // + the binomial table C(p, j), p < n <= 18, j < KM + 2 <= 10; two replicated rows
         // (one barrier per connection) if tuned so and they fit
         auto lds_for = [&](int db, i64 nbp) {
-            return ((size_t)db * NW * R + 16 + (size_t)nbp * NVS) * sizeof(T) +
+            return ((size_t)db * NW * R + 16) * sizeof(T) + (size_t)nbp * NVS * sizeof(AbSrcRow<T, NVS, NW>) +
                    (AB_SRC_BINOM + AB_SRC_CONN) * 4;
         };
         const size_t lds_max = std::min<size_t>(lds_budget(), 160 * 1024);
