@@ -168,6 +168,198 @@ static void decode_hamiltonian(const HostNet &N, DevNet &D) {
                 if (r != s) E(p, p, r, s) = E(r, s, p, p);
 }
 
+// ---- checking a decode against the network it came from ------------------------------------
+// Determinants as interleaved spin-orbital occupations: bit 2p = alpha p, bit 2p+1 = beta p
+// (src/include/ham/ham.hpp:519-551).
+static u64 interleave(u32 a, u32 b) {
+    u64 x = 0;
+    for (int p = 0; p < 32; ++p) x |= (u64)((a >> p) & 1u) << (2 * p) | (u64)((b >> p) & 1u) << (2 * p + 1);
+    return x;
+}
+
+// apply one ladder operator at spin-orbital pos to occupation x (Jordan-Wigner sign: parity of
+// the occupied spin-orbitals below pos); false if it annihilates the state
+static bool ladder_apply(u64 &x, int &par, int pos, bool create) {
+    const u64 bit = 1ull << pos;
+    if (((x & bit) != 0) == create) return false;
+    par += __builtin_popcountll(x & (bit - 1));
+    x ^= bit;
+    return true;
+}
+
+// <bra| H |ket> of the two-body operator held by D (ecore, h_pq, (pq|rs) chemist order), by the
+// Slater-Condon rules in the interleaved spin-orbital order, with the signs of the ladder
+// operators applied to ket:  H = ecore + sum h_PQ a+_P a_Q + 1/2 sum (PQ|RS) a+_P a+_R a_S a_Q
+static double sc_element(const DevNet &D, u64 bra, u64 ket) {
+    const int n = D.n;
+    auto h1 = [&](int P, int Q) { return ((P ^ Q) & 1) ? 0.0 : D.h[(size_t)(P >> 1) * n + (Q >> 1)]; };
+    auto g2 = [&](int P, int Q, int R, int S) {
+        if (((P ^ Q) & 1) || ((R ^ S) & 1)) return 0.0;
+        return D.eri[((((size_t)(P >> 1) * n + (Q >> 1)) * n + (R >> 1)) * n) + (S >> 1)];
+    };
+    const u64 diff = bra ^ ket;
+    const int nd = __builtin_popcountll(diff);
+    std::vector<int> occ;
+    for (u64 m = ket; m; m &= m - 1) occ.push_back(__builtin_ctzll(m));
+    if (nd == 0) {
+        double e = D.ecore;
+        for (int P : occ) e += h1(P, P);
+        for (int P : occ)
+            for (int R : occ)
+                if (P != R) e += 0.5 * (g2(P, P, R, R) - g2(P, R, R, P));
+        return e;
+    }
+    if (nd == 2) {
+        const int Q = __builtin_ctzll(ket & diff), P = __builtin_ctzll(bra & diff);
+        if (!(ket & diff) || !(bra & diff)) return 0.0;
+        u64 x = ket;
+        int par = 0;
+        ladder_apply(x, par, Q, false);
+        ladder_apply(x, par, P, true);
+        double v = h1(P, Q);
+        for (int R : occ)
+            if (R != Q) v += g2(P, Q, R, R) - g2(P, R, R, Q);
+        return (par & 1) ? -v : v;
+    }
+    if (nd == 4) {
+        const u64 an = ket & diff, cr = bra & diff;
+        if (__builtin_popcountll(an) != 2 || __builtin_popcountll(cr) != 2) return 0.0;
+        const int Q1 = __builtin_ctzll(an), Q2 = 63 - __builtin_clzll(an);
+        const int P1 = __builtin_ctzll(cr), P2 = 63 - __builtin_clzll(cr);
+        u64 x = ket;
+        int par = 0;
+        ladder_apply(x, par, Q1, false);
+        ladder_apply(x, par, Q2, false);
+        ladder_apply(x, par, P2, true);
+        ladder_apply(x, par, P1, true);
+        const double v = g2(P1, Q1, P2, Q2) - g2(P1, Q2, P2, Q1);
+        return (par & 1) ? -v : v;
+    }
+    return 0.0;
+}
+
+// deterministic pseudo-random subset of the bits of m (splitmix64 of the key)
+static u32 probe_subset(u32 m, u64 key) {
+    u64 z = key + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return m & (u32)(z ^ (z >> 32));
+}
+
+// The decode reads a few matrix elements per excitation pattern and assumes the network is a
+// spin-free two-body Hamiltonian with real 8-fold-symmetric integrals.  Any other network
+// (spin models, S^2 penalties, three-body terms, missing Jordan-Wigner strings) would decode into
+// a different operator, so every number- and spin-conserving pattern of up to two electrons
+// per spin is re-evaluated from the decoded integrals on probe determinants -- every
+// annihilated/created split of the pattern, with no, all, single and pseudo-random spectator
+// electrons of each spin -- and compared with the network's own element
+// (src/include/core/math.hpp:37-124).  Networks with groups outside those patterns are
+// rejected outright.
+static void verify_hamiltonian(const HostNet &N, const DevNet &D) {
+    const int n = N.norb;
+    const u32 full = n >= 32 ? 0xFFFFFFFFu : ((1u << n) - 1);
+    for (size_t g = 0; g < N.groups.size(); ++g) {
+        const int ka = __builtin_popcount(N.groups[g].ax), kb = __builtin_popcount(N.groups[g].bx);
+        HIPOTF_REQUIRE((N.groups[g].ax & ~full) == 0 && (N.groups[g].bx & ~full) == 0 && ka % 2 == 0 &&
+                           kb % 2 == 0 && ka + kb <= 4,
+                       "network group " + std::to_string(g) +
+                           " is not a number- and spin-conserving one- or two-body term; only two-body "
+                           "Hamiltonian networks are decoded (generic Pauli networks are not supported)");
+    }
+    double scale = std::fabs(D.ecore);
+    for (double v : D.h) scale = std::max(scale, std::fabs(v));
+    for (double v : D.eri) scale = std::max(scale, std::fabs(v));
+    const double tol = 1e-9 * std::max(1.0, scale);
+    // the sigma kernels use (pq|rs) = (qp|rs) = (rs|pq) (real orbitals): a spin-free operator
+    // without that symmetry (an S^2 penalty: (pq|qp) without (pq|pq)) is not representable
+    for (int p = 0; p < n; ++p)
+        for (int q = 0; q < n; ++q) {
+            HIPOTF_REQUIRE(std::fabs(D.h[p * n + q] - D.h[q * n + p]) <= tol,
+                           "network's one-body part is not symmetric (generic Pauli networks are not supported)");
+            for (int r = 0; r < n; ++r)
+                for (int s = 0; s < n; ++s) {
+                    const double v = D.eri[(((size_t)p * n + q) * n + r) * n + s];
+                    const double v1 = D.eri[(((size_t)q * n + p) * n + r) * n + s];
+                    const double v2 = D.eri[(((size_t)r * n + s) * n + p) * n + q];
+                    HIPOTF_REQUIRE(std::fabs(v - v1) <= tol && std::fabs(v - v2) <= tol,
+                                   "network is not a spin-free two-body Hamiltonian with real 8-fold symmetric "
+                                   "integrals (generic Pauli networks are not supported)");
+                }
+        }
+    double worst = 0.0;
+    u64 nprobe = 0;
+    auto check = [&](u32 ax, u32 bx, u32 a, u32 b) {
+        const double fn = N.element(ax, bx, a, b);
+        const double fs = sc_element(D, interleave(a ^ ax, b ^ bx), interleave(a, b));
+        const double d = std::fabs(fn - fs);
+        ++nprobe;
+        if (!(d <= tol)) {
+            char msg[320];
+            std::snprintf(msg, sizeof msg,
+                          "network is not a spin-free two-body Hamiltonian: element <a^%#x,b^%#x|H|a=%#x,b=%#x> "
+                          "is %.12g in the network and %.12g from the decoded integrals (generic Pauli networks "
+                          "are not supported)",
+                          ax, bx, a, b, fn, fs);
+            throw Error(msg);
+        }
+        worst = std::max(worst, d);
+    };
+    std::vector<int> bits;
+    auto subsets = [](u32 m, int k, std::vector<u32> &out) {  // subsets of m with k bits
+        out.clear();
+        std::vector<int> b;
+        for (u32 t = m; t; t &= t - 1) b.push_back(__builtin_ctz(t));
+        for (u32 s = 0; s < (1u << b.size()); ++s) {
+            if (__builtin_popcount(s) != k) continue;
+            u32 v = 0;
+            for (size_t i = 0; i < b.size(); ++i)
+                if ((s >> i) & 1) v |= 1u << b[i];
+            out.push_back(v);
+        }
+    };
+    std::vector<u32> pa_list, pb_list, sa_list, sb_list;
+    // every pattern: alpha / beta flip masks of 0, 2 or 4 orbitals, at most 4 in total
+    std::vector<u32> masks[5];
+    for (int k : {0, 2, 4}) subsets(full, k, masks[k]);
+    for (int ka : {0, 2, 4})
+        for (int kb : {0, 2, 4}) {
+            if (ka + kb > 4) continue;
+            for (u32 ax : masks[ka])
+                for (u32 bx : masks[kb]) {
+                    subsets(ax, ka / 2, pa_list);
+                    subsets(bx, kb / 2, pb_list);
+                    const u32 fa = full & ~ax, fb = full & ~bx;
+                    const int nrand = ka + kb == 0 ? 256 : 4;
+                    for (u32 anna : pa_list)
+                        for (u32 annb : pb_list) {
+                            check(ax, bx, anna, annb);
+                            check(ax, bx, anna | fa, annb | fb);
+                            if (ka + kb <= 2) {  // single spectators (the diagonal: also pairs)
+                                for (u32 t = fa; t; t &= t - 1) check(ax, bx, anna | (t & -t), annb);
+                                for (u32 t = fb; t; t &= t - 1) check(ax, bx, anna, annb | (t & -t));
+                            }
+                            if (ka + kb == 0)
+                                for (u32 s = fa; s; s &= s - 1)
+                                    for (u32 t = fb; t; t &= t - 1) check(ax, bx, (s & -s), (t & -t));
+                            if (ka + kb == 0)
+                                for (u32 s = fa; s; s &= s - 1)
+                                    for (u32 t = s & (s - 1); t; t &= t - 1) {
+                                        check(ax, bx, (s & -s) | (t & -t), 0);
+                                        check(ax, bx, 0, (s & -s) | (t & -t));
+                                    }
+                            for (int r = 0; r < nrand; ++r) {
+                                const u64 key = ((u64)ax << 32 | bx) * 1315423911ull + (u64)anna * 2654435761ull +
+                                                annb * 40503ull + (u64)r * 0x632BE59BD9B4E019ull;
+                                check(ax, bx, anna | probe_subset(fa, key), annb | probe_subset(fb, key ^ 0xA5A5A5A5ull));
+                            }
+                        }
+                }
+        }
+    (void)worst;
+    (void)nprobe;
+}
+
 // recognise each pool group as +-(T - T^dagger) for a product T of ladder operators
 static void decode_pool(const HostNet &N, DevNet &D) {
     const int n = N.norb;
@@ -242,6 +434,58 @@ static void decode_pool(const HostNet &N, DevNet &D) {
         }
         HIPOTF_REQUIRE(found, "pool group " + std::to_string(g) +
                                   " is not a fermionic excitation generator (generic networks unsupported)");
+    }
+}
+
+// <T-decoded generator> == network group on probe determinants: every split of the group's
+// flip pattern, with no, all and pseudo-random spectators.  G = T - T^dagger, T the ladder
+// product of offsets[g]..offsets[g+1] (creators first), so <x^f|G|x> = sign(T x) when T maps x
+// to x^f, -sign(T x^f) when T maps x^f to x, 0 otherwise.
+static void verify_pool(const HostNet &N, const DevNet &D) {
+    const int n = N.norb;
+    const u32 full = n >= 32 ? 0xFFFFFFFFu : ((1u << n) - 1);
+    auto apply_term = [&](const std::vector<std::pair<int, bool>> &ops, u64 x, u64 &out) -> int {
+        int par = 0;
+        for (int i = (int)ops.size() - 1; i >= 0; --i)
+            if (!ladder_apply(x, par, ops[i].first, ops[i].second)) return 0;
+        out = x;
+        return (par & 1) ? -1 : 1;
+    };
+    for (size_t g = 0; g < N.groups.size(); ++g) {
+        const HostGroup &G = N.groups[g];
+        const int h = (D.offsets[g + 1] - D.offsets[g]) / 2;
+        std::vector<std::pair<int, bool>> T, Td;
+        for (int i = D.offsets[g]; i < D.offsets[g + 1]; ++i) T.emplace_back(D.orbs[i], i - D.offsets[g] < h);
+        for (int i = (int)T.size() - 1; i >= 0; --i) Td.emplace_back(T[i].first, !T[i].second);
+        const u32 fa = full & ~G.ax, fb = full & ~G.bx;
+        for (u32 anna = G.ax;; anna = (anna - 1) & G.ax) {
+            for (u32 annb = G.bx;; annb = (annb - 1) & G.bx) {
+                for (int r = 0; r < 6; ++r) {
+                    const u64 key = (u64)g * 0x9E3779B97F4A7C15ull + (u64)anna * 977 + annb * 131 + r;
+                    const u32 sa = r == 0 ? 0u : r == 1 ? fa : probe_subset(fa, key);
+                    const u32 sb = r == 0 ? 0u : r == 1 ? fb : probe_subset(fb, key ^ 0x5Aull);
+                    const u32 a = anna | sa, b = annb | sb;
+                    const u64 ket = interleave(a, b), bra = interleave(a ^ G.ax, b ^ G.bx);
+                    u64 o = 0;
+                    double e = 0.0;
+                    int s = apply_term(T, ket, o);
+                    if (s && o == bra) e += s;
+                    s = apply_term(Td, ket, o);
+                    if (s && o == bra) e -= s;
+                    const double f = N.element(G.ax, G.bx, a, b);
+                    if (!(std::fabs(f - e) <= 1e-9)) {
+                        char msg[256];
+                        std::snprintf(msg, sizeof msg,
+                                      "pool group %zu is not a fermionic excitation generator T - T^dagger: element "
+                                      "at a=%#x b=%#x is %.12g, expected %.12g (generic networks unsupported)",
+                                      g, a, b, f, e);
+                        throw Error(msg);
+                    }
+                }
+                if (annb == 0) break;
+            }
+            if (anna == 0) break;
+        }
     }
 }
 
@@ -370,8 +614,13 @@ void *build_networkdev_f64(void *host_net) {
     const HostNet &N = *static_cast<HostNet *>(host_net);
     auto D = std::make_unique<DevNet>();
     bool has_diag = N.by_x.count(0) > 0;
-    if (has_diag) decode_hamiltonian(N, *D);
-    else decode_pool(N, *D);
+    if (has_diag) {
+        decode_hamiltonian(N, *D);
+        verify_hamiltonian(N, *D);
+    } else {
+        decode_pool(N, *D);
+        verify_pool(N, *D);
+    }
     return D.release();
     API_CATCH(nullptr)
 }

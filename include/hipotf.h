@@ -56,7 +56,9 @@ void *build_basisdev_f64(void *host_basis);               /* cuotf.cu:25-30 */
 void destroy_basisdev_f64(void *basis);                   /* cuotf.cu:14-23 */
 /* build_networkdev_f64 (cuotf.cu:43-48): decodes a host network from build_network_otf_f64.
  * A network with a diagonal group is read as a two-body Hamiltonian (integrals recovered by
- * probing matrix elements); otherwise every group must be +-(T - T^dagger) of one excitation. */
+ * probing matrix elements); otherwise every group must be +-(T - T^dagger) of one excitation.
+ * The decode is re-checked against the network's elements on probe determinants; a network it
+ * does not represent exactly (spin models, S^2 penalties, non-two-body terms) returns NULL. */
 void *build_networkdev_f64(void *host_net);
 void destroy_networkdev_f64(void *net);                   /* cuotf.cu:32-41 */
 /* hvec_cuda (cuotf.cu:50-60): dst = H src (dst overwritten, hvec.cuh:439) */

@@ -1,0 +1,156 @@
+"""build_networkdev_f64 on networks that are not plain molecular Hamiltonians.
+
+The device handle decodes a reference SVD network (jl/network.jl:158-339) into integrals and
+checks the decode against the network's own elements (src/include/core/math.hpp:37-124) on
+probe determinants.  Networks the decode cannot represent must come back NULL with an error
+instead of silently becoming a different operator: the S^2 penalty and the spin models of
+jl/hamiltonian.jl:405-484.  An (N - n)^2 particle-number penalty (apply_constraint,
+jl/hamiltonian.jl:429-435) is a spin-free two-body operator with 8-fold symmetric integrals,
+so H + mu (N - n)^2 decodes -- and the GPU test checks its sigma against the oracle, which
+evaluates the network itself.
+"""
+import numpy as np
+import pytest
+
+import binsim_amd.lib as L
+from binsim_amd import otf
+from oracle import glue
+from oracle import oracle as O
+from tests.helpers import integrals, oracle_system, rel_err, seeded_vector
+
+
+def _as_dict(terms):
+    axs, bxs, azs, bzs, cs = terms
+    d = {}
+    for ax, bx, az, bz, c in zip(axs, bxs, azs, bzs, cs):
+        key = (glue.unzip_even(int(ax)) | glue.unzip_odd(int(bx)), glue.unzip_even(int(az)) | glue.unzip_odd(int(bz)))
+        d[key] = d.get(key, 0.0) + float(c)
+    return d
+
+
+def _add(d, pauli, w=1.0):
+    for x, z, c in pauli:
+        d[(x, z)] = d.get((x, z), 0.0) + w * c
+
+
+def _network(d, tol=1e-12):
+    keys = sorted(k for k, v in d.items() if abs(v) > tol)
+    arr = lambda f: np.array([f(k) for k in keys], dtype=np.uint32)  # noqa: E731
+    terms = (arr(lambda k: glue.zip_even(k[0])), arr(lambda k: glue.zip_odd(k[0])),
+             arr(lambda k: glue.zip_even(k[1])), arr(lambda k: glue.zip_odd(k[1])),
+             np.array([d[k] for k in keys], dtype=np.float64))
+    return O.Network(glue.compress_by_svd(terms))
+
+
+def _number_op(n):
+    out = []
+    for pos in range(2 * n):
+        out += glue.fermion_operator([(pos, 1), (pos, 0)])
+    return glue._reduce(out, 0.0)
+
+
+def _ham_dict(n):
+    _, _, h1, g2 = integrals(n)
+    return _as_dict(glue.jw_hamiltonian_terms(h1, g2, n))
+
+
+def _number_penalty(n, nelec, mu):
+    d = _ham_dict(n)
+    N = _number_op(n)
+    shifted = glue._reduce(N + [(0, 0, -float(nelec))], 0.0)
+    _add(d, glue.multiply([shifted, shifted]), mu)
+    return _network(d)
+
+
+def _s2_penalty(n, mu):
+    # S^2 = S_- S_+ + S_z (S_z + 1), S_+ = sum_p a+_{p alpha} a_{p beta}
+    d = _ham_dict(n)
+    sp, sm, sz = [], [], []
+    for p in range(n):
+        sp += glue.fermion_operator([(2 * p, 1), (2 * p + 1, 0)])
+        sm += glue.fermion_operator([(2 * p + 1, 1), (2 * p, 0)])
+        sz += [(x, z, 0.5 * c) for x, z, c in glue.fermion_operator([(2 * p, 1), (2 * p, 0)])]
+        sz += [(x, z, -0.5 * c) for x, z, c in glue.fermion_operator([(2 * p + 1, 1), (2 * p + 1, 0)])]
+    sp, sm, sz = (glue._reduce(t, 0.0) for t in (sp, sm, sz))
+    s2 = glue.multiply([sm, sp]) + glue.multiply([sz, glue._reduce(sz + [(0, 0, 1.0)], 0.0)])
+    _add(d, glue._reduce(s2, 0.0), mu)
+    return _network(d)
+
+
+def _qubit_chain(nq, zz=1.0, xx=0.0, x_field=0.0):
+    """Ising / Heisenberg chains of jl/hamiltonian.jl:438-484 on nq qubits (interleaved)."""
+    d = {}
+    for i in range(nq - 1):
+        j = i + 1
+        if zz:
+            _add(d, [(0, (1 << i) | (1 << j), zz)])
+        if xx:
+            _add(d, [((1 << i) | (1 << j), 0, xx)])
+    for i in range(nq):
+        if x_field:
+            _add(d, [(1 << i, 0, x_field)])
+    return _network(d)
+
+
+def _decode(n, net):
+    return otf.HipOTF.from_network(n, net.flat)
+
+
+def test_molecular_network_decodes():
+    assert _decode(6, O.hamiltonian_network(*integrals(6)[2:], 6)).ptr
+
+
+def test_number_penalty_decodes():
+    assert _decode(6, _number_penalty(6, 6, 0.7)).ptr
+
+
+@pytest.mark.parametrize("make", [
+    lambda: _s2_penalty(6, 0.5),
+    lambda: _qubit_chain(12, zz=-1.0),                   # Ising ZZ chain, h = 0: diagonal only
+    lambda: _qubit_chain(12, zz=-1.0, x_field=-0.5),     # transverse-field Ising
+    lambda: _qubit_chain(12, zz=1.0, xx=1.0),            # XXZ-type Heisenberg (no Y)
+], ids=["s2_penalty", "ising_zz", "ising_tf", "heisenberg"])
+def test_non_two_body_networks_are_rejected(make):
+    with pytest.raises(L.HipOTFError, match="not supported"):
+        _decode(6, make())
+
+
+def test_wrong_jordan_wigner_strings_are_rejected():
+    # a hopping term without its Z string between the two alpha orbitals: a valid pattern
+    # whose elements disagree once an electron sits between them
+    d = _ham_dict(6)
+    _add(d, [((1 << 0) | (1 << 6), 0, 0.3)])
+    with pytest.raises(L.HipOTFError, match="spin-free two-body"):
+        _decode(6, _network(d))
+
+
+def test_pool_with_broken_generator_is_rejected():
+    from binsim_amd.ansatz import ucc_spin_orbitals
+    sos = ucc_spin_orbitals(6, 3, 3)[:6]
+    gens = [glue.ucc_generator(so) for so in sos]
+    axs, bxs, azs, bzs, cs = gens[2]
+    gens[2] = (axs, bxs, azs, bzs, np.where(np.arange(len(cs)) == 0, 2 * cs, cs))
+    net = O.Network(glue.compress_pool_by_svd(gens))
+    with pytest.raises(L.HipOTFError, match="generator"):
+        _decode(6, net)
+    ok = O.Network(glue.compress_pool_by_svd([glue.ucc_generator(so) for so in sos]))
+    assert _decode(6, ok).ptr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,na,nb", [(6, 3, 3), (6, 2, 4), (8, 4, 4)])
+def test_number_penalty_sigma_matches_oracle(torch_cuda, n, na, nb):
+    """H + 0.7 (N - 6)^2 through the drop-in network path equals the oracle's network sigma."""
+    from tests.conftest import gpu_vec, host
+    net = _number_penalty(n, 6, 0.7)
+    S = O.System(n, na, nb, ham=net)
+    x = seeded_vector(S.dim, 11)
+    ref = S.hvec(x)
+    B = otf.HipBasisManager(otf.BasisManager(n, na, nb))
+    F = otf.HipOTF_Functions(B, ham=_decode(n, net))
+    xd = gpu_vec(torch_cuda, x)
+    yd = torch_cuda.empty_like(xd)
+    F.hvec(xd, yd)
+    assert rel_err(host(yd), ref) < 1e-12
+    # the penalty really is in the operator: it differs from the plain Hamiltonian's sigma
+    assert rel_err(oracle_system(n, na, nb).hvec(x), ref) > 1e-3 or na + nb == 6
