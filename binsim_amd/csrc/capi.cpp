@@ -814,8 +814,8 @@ int hipotf_vqe_energy_grad(void *basis, void *ham, void *pool, int64_t nparams, 
     const double *src = psi;
     double *dst = lam;
     sigma(H, 1, &src, &dst, 0, st);
-    DevBuf dres((size_t)(nparams + 4) * sizeof(double));
-    double *d = dres.as<double>();
+    P->vqe_res.ensure((size_t)(nparams + 4) * sizeof(double));  // kept across calls
+    double *d = P->vqe_res.as<double>();
     dot_dev(dim, psi, psi, d + nparams, st);
     dot_dev(dim, lam, lam, d + nparams + 1, st);
     dot_dev(dim, psi, lam, d + nparams + 2, st);
@@ -828,6 +828,37 @@ int hipotf_vqe_energy_grad(void *basis, void *ham, void *pool, int64_t nparams, 
     out3[0] = e;
     out3[1] = std::max(0.0, rnorm / lnorm - e * e);
     out3[2] = lnorm;
+    return 0;
+    API_CATCH(1)
+}
+
+// energy only (the line-search / finite-difference evaluations of jl/vqe.jl): forward sweep,
+// lam = H psi, energy and variance; psi is left as the ansatz state U(theta)|ref>
+int hipotf_vqe_energy(void *basis, void *ham, void *pool, int64_t nparams, const int64_t *idxs,
+                      const double *thetas, double *psi, double *lam, double *out3) {
+    API_TRY
+    DevBasis *B = as_basis(basis);
+    DevHam *H = as_net(ham)->ham_for(B);
+    DevPool *P = as_net(pool)->pool_for(B);
+    HIPOTF_REQUIRE(psi && lam && out3 && (nparams == 0 || (idxs && thetas)), "NULL argument");
+    const i64 dim = B->host.dim;
+    hipStream_t st = S0();
+    fused_sweeps(P, nparams, idxs, thetas, psi, lam, true, false, nullptr, st);
+    const double *src = psi;
+    double *dst = lam;
+    sigma(H, 1, &src, &dst, 0, st);
+    P->vqe_res.ensure(4 * sizeof(double));
+    double *d = P->vqe_res.as<double>();
+    dot_dev(dim, psi, psi, d, st);
+    dot_dev(dim, lam, lam, d + 1, st);
+    dot_dev(dim, psi, lam, d + 2, st);
+    double h[3];
+    HIPOTF_CHECK(hipMemcpyAsync(h, d, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPOTF_CHECK(hipStreamSynchronize(st));
+    const double e = h[2] / h[0];
+    out3[0] = e;
+    out3[1] = std::max(0.0, h[1] / h[0] - e * e);
+    out3[2] = h[0];
     return 0;
     API_CATCH(1)
 }
@@ -852,7 +883,7 @@ int hipotf_davidson_f64(void *basis, void *ham, int nroots, int mode, int block,
     info[4] = o.seconds;
     info[5] = o.refine_iter;
     info[6] = o.converged;
-    return o.converged ? 0 : 2;
+    return o.converged == 1 ? 0 : o.converged == 2 ? 3 : 2;
     API_CATCH(1)
 }
 

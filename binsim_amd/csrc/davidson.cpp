@@ -36,6 +36,7 @@ void block_dots(i64 N, int na, const double *const *A, int nbv, const double *co
                 hipStream_t st);
 void block_subtract(i64 N, int m, const double *const *V, const double *P, int nt, double *const *T,
                     hipStream_t st);
+void rotate_inplace(i64 N, int m, int keep, const double *C, double *const *X, hipStream_t st);
 
 // eigen-decomposition of a symmetric m x m matrix (row-major a), ascending eigenvalues,
 // eigenvectors in the columns of v (row-major)
@@ -116,11 +117,34 @@ DavidsonOut davidson(DevHam *H, int nroots, int mode, int block, double tol, dou
     HIPOTF_REQUIRE(maxspace >= 2 * k || maxspace == N, "maxspace must be >= 2 * block");
     DavidsonOut out;
 
+    // device memory of the whole solve, checked before anything is allocated: 2 * maxspace
+    // subspace vectors (V, AV), k residuals, the diagonal, and the sigma work buffers (4 per
+    // basis, dim * vectors per launch * word size; grown on first use).  The loop itself
+    // allocates nothing: restarts and the FP32 -> FP64 switch combine the subspace in place.
+    {
+        const int nv_sigma = mode == 0 ? 1 : std::min(k, 8);
+        const size_t vec = (size_t)N * sizeof(double);
+        const size_t need_vec = vec * (2 * (size_t)maxspace + k + 1);
+        const size_t need_sig = 4 * vec * nv_sigma;  // FP64 phase: 4 buffers of dim * nv doubles
+        const size_t have_sig = std::min(std::min(H->basis->xt.bytes, H->basis->xtT.bytes),
+                                         std::min(H->basis->s1.bytes, H->basis->s2T.bytes));
+        const size_t grow = need_sig / 4 > have_sig ? 4 * (need_sig / 4 - have_sig) : 0;
+        size_t free_b = 0, total_b = 0;
+        HIPOTF_CHECK(hipMemGetInfo(&free_b, &total_b));
+        if (need_vec + grow > free_b) {
+            char msg[320];
+            std::snprintf(msg, sizeof msg,
+                          "davidson: needs %.1f GB of device memory (%d + %d subspace vectors, %d residuals, "
+                          "diagonal, %.1f GB more sigma work space) but %.1f GB are free; lower maxspace or block",
+                          (need_vec + grow) / 1e9, maxspace, maxspace, k, grow / 1e9, free_b / 1e9);
+            throw Error(msg);
+        }
+    }
     std::vector<DevBuf> Vb(maxspace), AVb(maxspace), R(k);
     for (auto &b : Vb) b.alloc(N * sizeof(double));
     for (auto &b : AVb) b.alloc(N * sizeof(double));
     for (auto &b : R) b.alloc(N * sizeof(double));
-    DevBuf diag(N * sizeof(double)), tmp(N * sizeof(double) * (size_t)k);
+    DevBuf diag(N * sizeof(double));
     HIPOTF_CHECK(hipMemsetAsync(diag.p, 0, N * sizeof(double), st));
     diag_elements(H, diag.as<double>(), st);
     std::vector<double *> V(maxspace), AV(maxspace);
@@ -217,6 +241,23 @@ DavidsonOut davidson(DevHam *H, int nroots, int mode, int block, double tol, dou
     std::vector<double> w, C, best(3, 1e300);
     std::vector<double> rn(k);
     int stall = 0;
+    // the latest Ritz vectors when the loop ends unconverged: V[0..nroots) themselves after a
+    // restart or the FP32 -> FP64 switch in the last iteration, else the last coefficients over
+    // V[0..coef_m) (vectors added after them sit at higher positions)
+    std::vector<std::vector<double>> coef_last;
+    bool ritz_in_V = false;
+    auto write_current = [&]() {
+        if (!vecs) return;
+        for (int i = 0; i < nroots; ++i) {
+            if (ritz_in_V || coef_last.empty())
+                HIPOTF_CHECK(hipMemcpyAsync(vecs + (size_t)i * N, V[std::min(i, m - 1)], N * sizeof(double),
+                                            hipMemcpyDeviceToDevice, st));
+            else
+                ritz_vector(N, (int)coef_last[std::min<size_t>(i, coef_last.size() - 1)].size(),
+                            coef_last[std::min<size_t>(i, coef_last.size() - 1)].data(),
+                            (const double *const *)V.data(), vecs + (size_t)i * N, st);
+        }
+    };
     for (int it = 1; it <= maxiter; ++it) {
         out.iterations = it;
         std::vector<double> a((size_t)m * m);
@@ -228,6 +269,8 @@ DavidsonOut davidson(DevHam *H, int nroots, int mode, int block, double tol, dou
         double rmax = 0.0;  // over the wanted roots
         for (int i = 0; i < kk; ++i)
             for (int j = 0; j < m; ++j) coef[i][j] = C[(size_t)j * m + i];
+        coef_last = coef;
+        ritz_in_V = false;
         if (mode == 1) {
             // all kk residuals in one pass over the subspace
             std::vector<double> Cm((size_t)m * kk);
@@ -264,19 +307,14 @@ DavidsonOut davidson(DevHam *H, int nroots, int mode, int block, double tol, dou
         }
         if (rmax < cur_tol) {
             if (phase == 1) {
-                // the kk lowest Ritz vectors -> new subspace, FP64 sigma from here on
+                // the kk lowest Ritz vectors -> new subspace (combined in place), FP64 sigma from
+                // here on
                 {
                     std::vector<double> Cm((size_t)m * kk);
-                    std::vector<double *> Tp(kk);
                     for (int j = 0; j < m; ++j)
                         for (int i = 0; i < kk; ++i) Cm[(size_t)j * kk + i] = coef[i][j];
-                    for (int i = 0; i < kk; ++i) Tp[i] = tmp.as<double>() + (size_t)i * N;
-                    block_combine(N, m, kk, Cm.data(), nullptr, (const double *const *)V.data(), nullptr,
-                                  Tp.data(), nullptr, st);
+                    rotate_inplace(N, m, kk, Cm.data(), V.data(), st);
                 }
-                for (int i = 0; i < kk; ++i)
-                    HIPOTF_CHECK(hipMemcpyAsync(V[i], tmp.as<double>() + (size_t)i * N, N * sizeof(double),
-                                                hipMemcpyDeviceToDevice, st));
                 m = 0;
                 for (int i = 0; i < kk; ++i) {
                     mgs(N, m, (const double *const *)V.data(), V[i], st);
@@ -284,6 +322,7 @@ DavidsonOut davidson(DevHam *H, int nroots, int mode, int block, double tol, dou
                     ++m;
                 }
                 phase = 0;
+                ritz_in_V = true;
                 out.refine_iter = it;
                 apply(0, m);
                 set_cols(0, m);
@@ -307,33 +346,14 @@ DavidsonOut davidson(DevHam *H, int nroots, int mode, int block, double tol, dou
             std::vector<std::vector<double>> ck(keep, std::vector<double>(m));
             for (int i = 0; i < keep; ++i)
                 for (int j = 0; j < m; ++j) ck[i][j] = C[(size_t)j * m + i];
-            DevBuf nv((size_t)keep * N * sizeof(double)), nav((size_t)keep * N * sizeof(double));
-            if (mode == 1) {
-                std::vector<double> Cm((size_t)m * keep);
-                std::vector<double *> P1(keep), P2(keep);
-                for (int j = 0; j < m; ++j)
-                    for (int i = 0; i < keep; ++i) Cm[(size_t)j * keep + i] = ck[i][j];
-                for (int i = 0; i < keep; ++i) {
-                    P1[i] = nv.as<double>() + (size_t)i * N;
-                    P2[i] = nav.as<double>() + (size_t)i * N;
-                }
-                block_combine(N, m, keep, Cm.data(), nullptr, (const double *const *)V.data(), nullptr, P1.data(),
-                              nullptr, st);
-                block_combine(N, m, keep, Cm.data(), nullptr, (const double *const *)AV.data(), nullptr,
-                              P2.data(), nullptr, st);
-            } else {
-                for (int i = 0; i < keep; ++i) {
-                    ritz_vector(N, m, ck[i].data(), (const double *const *)V.data(), nv.as<double>() + (size_t)i * N, st);
-                    ritz_vector(N, m, ck[i].data(), (const double *const *)AV.data(), nav.as<double>() + (size_t)i * N, st);
-                }
-            }
-            for (int i = 0; i < keep; ++i) {
-                HIPOTF_CHECK(hipMemcpyAsync(V[i], nv.as<double>() + (size_t)i * N, N * sizeof(double),
-                                            hipMemcpyDeviceToDevice, st));
-                HIPOTF_CHECK(hipMemcpyAsync(AV[i], nav.as<double>() + (size_t)i * N, N * sizeof(double),
-                                            hipMemcpyDeviceToDevice, st));
-            }
+            // V_i, AV_i <- the Ritz combinations, in place (no second copy of the kept vectors)
+            std::vector<double> Cm((size_t)m * keep);
+            for (int j = 0; j < m; ++j)
+                for (int i = 0; i < keep; ++i) Cm[(size_t)j * keep + i] = ck[i][j];
+            rotate_inplace(N, m, keep, Cm.data(), V.data(), st);
+            rotate_inplace(N, m, keep, Cm.data(), AV.data(), st);
             m = keep;
+            ritz_in_V = true;
             std::fill(heff.begin(), heff.end(), 0.0);
             set_cols(0, m);
         }
@@ -384,17 +404,16 @@ DavidsonOut davidson(DevHam *H, int nroots, int mode, int block, double tol, dou
             ++m;
         }
         if (m == first) {
-            // breakdown: no new direction (jl/davidson.jl:135-143)
-            if (vecs)
-                for (int i = 0; i < nroots; ++i)
-                    ritz_vector(N, std::min(m, (int)coef[i].size()), coef[i].data(),
-                                (const double *const *)V.data(), vecs + (size_t)i * N, st);
+            // breakdown: no new direction although the residual is above tol
+            // (jl/davidson.jl:135-143 returns the current Ritz vector)
+            write_current();
             out.converged = 2;
             break;
         }
         apply(first, m - first);
         set_cols(first, m - first);
     }
+    if (out.converged == 0) write_current();  // maxiter: the latest estimates
     HIPOTF_CHECK(hipDeviceSynchronize());
     out.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return out;

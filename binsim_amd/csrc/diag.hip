@@ -495,6 +495,58 @@ void block_combine(i64 N, int m, int k, const double *C, const double *w, const 
     }
 }
 
+// In place X_r <- sum_j C[j][r] X_j for r < keep <= m <= DMAX: the subspace restart without a
+// second copy of the kept vectors.  Each thread loads all m elements of its index before it
+// writes any, and no two threads share an index, so reading and writing the same vectors is
+// safe.  C sits in LDS (uniform reads, broadcast).  M = m rounded up to a multiple of 8.
+struct PtrMut64 { double *p[DMAX]; };
+
+template <int M>
+__global__ __launch_bounds__(256) void k_rotate_inplace(PtrMut64 X, int m, int keep, const double *__restrict__ C,
+                                                        i64 n) {
+    __shared__ double cs[M * DMAX];
+    for (int t = threadIdx.x; t < m * keep; t += blockDim.x) cs[(t / keep) * DMAX + t % keep] = C[t];
+    __syncthreads();
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
+        double x[M];
+#pragma unroll
+        for (int j = 0; j < M; ++j) x[j] = j < m ? X.p[j][i] : 0.0;
+        for (int r = 0; r < keep; ++r) {
+            double acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < M; ++j)
+                if (j < m) acc += cs[j * DMAX + r] * x[j];
+            X.p[r][i] = acc;
+        }
+    }
+}
+
+// X_r <- sum_j C[j*keep + r] X_j in place (C host row-major [m][keep])
+void rotate_inplace(i64 N, int m, int keep, const double *C, double *const *X, hipStream_t st) {
+    HIPOTF_REQUIRE(m >= 1 && m <= DMAX && keep >= 1 && keep <= m, "rotate_inplace: bad sizes");
+    auto &S = bscratch();
+    HIPOTF_CHECK(hipMemcpyAsync(S.coef.p, C, (size_t)m * keep * sizeof(double), hipMemcpyHostToDevice, st));
+    PtrMut64 P{};
+    for (int j = 0; j < m; ++j) P.p[j] = X[j];
+    const int mm = (m + 7) / 8 * 8;
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(nblk_block(N)), dim3(256), 0, st, P, m, keep, S.coef.as<double>(), N);
+    };
+    switch (mm) {
+    case 8: go(k_rotate_inplace<8>); break;
+    case 16: go(k_rotate_inplace<16>); break;
+    case 24: go(k_rotate_inplace<24>); break;
+    case 32: go(k_rotate_inplace<32>); break;
+    case 40: go(k_rotate_inplace<40>); break;
+    case 48: go(k_rotate_inplace<48>); break;
+    case 56: go(k_rotate_inplace<56>); break;
+    default: go(k_rotate_inplace<64>); break;
+    }
+    HIPOTF_CHECK(hipGetLastError());
+    // the coefficient buffer is rewritten by the next block call: let this one finish reading it
+    HIPOTF_CHECK(hipStreamSynchronize(st));
+}
+
 // G[a][b] = A_a . B_b into host row-major [na][nb] (na <= DMAX, nb <= BK)
 void block_dots(i64 N, int na, const double *const *A, int nbv, const double *const *B, double *G,
                 hipStream_t st) {

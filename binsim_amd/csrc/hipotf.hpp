@@ -245,6 +245,7 @@ struct DevPool {
     DevBuf scal;                   // a few device doubles
     DevBuf segdesc, segpart;       // fused sweeps: per-position descriptors, per-block partials
     DevBuf tl, tr;                 // fused sweeps: beta-major copies of the two state vectors
+    DevBuf vqe_res;                // fused VQE objective: per-parameter results + norms (grown, kept)
 };
 
 // forward expm sweep and/or adjoint backgrad sweep over positions idxs[0..n) with fused

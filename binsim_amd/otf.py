@@ -193,6 +193,17 @@ class HipOTF_Functions:
         L.check(rc == 0, "hipotf_vqe_energy_grad")
         return float(out3[0]), g, float(out3[1]), float(out3[2])
 
+    def vqe_energy(self, idxs, thetas, psi, lam):
+        """Energy only (no gradient): returns (energy, dH2, |psi|^2); psi ends as U(theta)|ref>."""
+        idx = _np(idxs, np.int64)
+        th = _np(thetas, np.float64)
+        out3 = np.empty(3)
+        rc = L.load().hipotf_vqe_energy(
+            self.basis.ptr, self.ham.ptr, self.pool.ptr, len(th), idx.ctypes.data_as(L._i64p),
+            th.ctypes.data_as(L._dp), _ptr(psi), _ptr(lam), out3.ctypes.data_as(L._dp))
+        L.check(rc == 0, "hipotf_vqe_energy")
+        return float(out3[0]), float(out3[1]), float(out3[2])
+
 
 def sync_device():
     L.call_void("sync_device_cuda")

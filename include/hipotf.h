@@ -124,6 +124,11 @@ int hipotf_hvec_block(void *basis, void *net, int nvec, const double *const *src
 int hipotf_vqe_energy_grad(void *basis, void *ham, void *pool, int64_t nparams, const int64_t *idxs,
                            const double *thetas, double *psi, double *lam, double *grads,
                            double *out3);
+/* energy only (the evaluations of jl/vqe.jl:1-21 without the gradient: line searches,
+ * finite-difference checks): forward sweep, lam = H psi, out3 as above; psi: in = reference
+ * state, out = U(theta)|ref> (not uncomputed). */
+int hipotf_vqe_energy(void *basis, void *ham, void *pool, int64_t nparams, const int64_t *idxs,
+                      const double *thetas, double *psi, double *lam, double *out3);
 
 /* Davidson solve on the device (outer loop of jl/davidson.jl:1-185 extended to nroots roots;
  * no reference counterpart runs on the GPU).  mode 0: FP64, one FP64 sigma per correction
@@ -136,7 +141,11 @@ int hipotf_vqe_energy_grad(void *basis, void *ham, void *pool, int64_t nparams, 
  * converged); vecs: device [nroots*dim] or NULL; info: host [7] = {iterations, fp64 sigma
  * vectors, fp32 sigma vectors, max residual, seconds, switch iteration, converged}.
  * maxspace <= 0 selects 23 for one root (ncv + 20) and 8*nroots in mode 0, 16*block in
- * mode 1 (at most 64).  Returns 0 when converged, 2 when maxiter was reached, 1 on error. */
+ * mode 1 (at most 64).  The subspace, residuals and sigma work space are checked against the
+ * free device memory before the loop (error if they do not fit); the loop allocates nothing.
+ * Returns 0 when converged, 2 when maxiter was reached, 3 on breakdown (no new direction while
+ * a residual is still above tol; info[6] = 2), 1 on error; with 2 and 3, energies and vecs
+ * hold the latest Ritz pairs. */
 int hipotf_davidson_f64(void *basis, void *ham, int nroots, int mode, int block, double tol,
                         double switch_tol, int maxiter, int maxspace, int verbose, const double *guess,
                         double *energies, double *vecs, double *info);
