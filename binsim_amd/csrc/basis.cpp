@@ -13,6 +13,7 @@
 namespace hipotf {
 
 static thread_local std::string g_last_error;
+thread_local bool g_forbid_alloc = false;
 void set_last_error(const std::string &msg) { g_last_error = msg; }
 void clear_last_error() { g_last_error.clear(); }
 const char *last_error_cstr() { return g_last_error.c_str(); }

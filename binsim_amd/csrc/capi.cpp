@@ -88,6 +88,14 @@ struct DevNet {
         hams[b->uid].reset(H);
         return H;
     }
+    // the Hamiltonian's tables for b if they exist (no allocation), else an error
+    DevHam *ham_prepared(DevBasis *b) {
+        HIPOTF_REQUIRE(kind == 1, "operator handle is not a Hamiltonian");
+        auto it = hams.find(b->uid);
+        HIPOTF_REQUIRE(it != hams.end(), "the Hamiltonian's device tables for this basis are not built: call "
+                                         "hipotf_prepare (or hvec_cuda) once first");
+        return it->second.get();
+    }
     DevPool *pool_for(DevBasis *b) {
         HIPOTF_REQUIRE(kind == 2, "operator handle is not a generator pool");
         auto it = pools.find(b->uid);
@@ -722,6 +730,20 @@ void hvec_cuda(void *basis, void *net, const double *src, double *dst) {
     DevHam *H = as_net(net)->ham_for(B);
     sigma(H, 1, &src, &dst, 0, S0());
     API_CATCH_VOID
+}
+
+// allocation-free hvec_cuda (for callers that cannot allocate device memory); the limit and
+// the tables it needs are checked before any work is issued
+int hipotf_hvec_noalloc(void *basis, void *net, const double *src, double *dst) {
+    API_TRY
+    DevBasis *B = as_basis(basis);
+    HIPOTF_REQUIRE(B->host.dim <= NOALLOC_MAX_DIM,
+                   "hipotf_hvec_noalloc: " + std::to_string(B->host.dim) + " determinants exceed the limit of " +
+                       std::to_string(NOALLOC_MAX_DIM));
+    DevHam *H = as_net(net)->ham_prepared(B);
+    sigma_noalloc(H, src, dst, S0());
+    return 0;
+    API_CATCH(1)
 }
 
 int hipotf_hvec_block(void *basis, void *net, int nvec, const double *const *src, double *const *dst,

@@ -49,6 +49,15 @@ struct Error : std::runtime_error {
         if (!(cond)) throw ::hipotf::Error(msg);    \
     } while (0)
 
+// Set while an allocation-free entry point runs: any device allocation attempted underneath
+// (a table that would be built on first use) throws instead of calling hipMalloc.
+extern thread_local bool g_forbid_alloc;
+struct ForbidAlloc {
+    bool prev;
+    ForbidAlloc() : prev(g_forbid_alloc) { g_forbid_alloc = true; }
+    ~ForbidAlloc() { g_forbid_alloc = prev; }
+};
+
 // RAII device allocation
 struct DevBuf {
     void *p = nullptr;
@@ -64,6 +73,9 @@ struct DevBuf {
     }
     ~DevBuf() { release(); }
     void alloc(size_t n) {
+        if (g_forbid_alloc)
+            throw Error("a device table this call needs is not built yet (an allocation-free entry point "
+                        "never allocates): call hipotf_prepare or the allocating entry point once first");
         release();
         if (n == 0) return;
         HIPOTF_CHECK(hipMalloc(&p, n));
@@ -217,6 +229,12 @@ constexpr int TUNING_FIELDS = 8;
 void sigma(DevHam *H, int nv, const double *const *src, double *const *dst, int prec,
            hipStream_t st);
 void diag_elements(DevHam *H, double *diags, hipStream_t st);  // accumulates
+// one FP64 sigma with the four work buffers carved from the library's static device scratch
+// array (no device allocation; every table must already exist); at most
+// NOALLOC_MAX_DIM determinants
+constexpr size_t NOALLOC_SCRATCH_BYTES = 64ull << 20;
+constexpr i64 NOALLOC_MAX_DIM = (i64)(NOALLOC_SCRATCH_BYTES / 32);
+void sigma_noalloc(DevHam *H, const double *src, double *dst, hipStream_t st);
 
 // ------------------------------------------------------------------------------------------
 // generator pool (UCC excitations G = T - T^dagger, jl/ansatz.jl:230-276)

@@ -16,6 +16,8 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 namespace hipotf {
 
@@ -828,18 +830,30 @@ static void launch_ab(DevHam *H, const T *xt, T *s1, int nv, const T *eri, hipSt
     }
 }
 
+// work: NULL (the basis's own work buffers, grown as needed) or 4 consecutive 256-byte aligned
+// regions of round_up(dim * NV * sizeof(T), 256) bytes each
 template <typename T, int NV>
-static void sigma_group(DevHam *H, const SrcPack &src, const DstPack &dst, hipStream_t st) {
+static void sigma_group(DevHam *H, const SrcPack &src, const DstPack &dst, hipStream_t st,
+                        unsigned char *work = nullptr) {
     DevBasis *B = H->basis;
     const DevSpin &A = *B->pA, &Bs = *B->pB;
     const i64 NA = A.N, NB = Bs.N, dim = NA * NB;
     if (dim == 0) return;
     const size_t bytes = (size_t)dim * NV * sizeof(T);
-    B->xt.ensure(bytes);
-    B->xtT.ensure(bytes);
-    B->s1.ensure(bytes);
-    B->s2T.ensure(bytes);
-    T *xt = B->xt.as<T>(), *xtT = B->xtT.as<T>(), *s1 = B->s1.as<T>(), *s2T = B->s2T.as<T>();
+    T *xt, *xtT, *s1, *s2T;
+    if (work) {
+        const size_t stride = (bytes + 255) / 256 * 256;
+        xt = reinterpret_cast<T *>(work);
+        xtT = reinterpret_cast<T *>(work + stride);
+        s1 = reinterpret_cast<T *>(work + 2 * stride);
+        s2T = reinterpret_cast<T *>(work + 3 * stride);
+    } else {
+        B->xt.ensure(bytes);
+        B->xtT.ensure(bytes);
+        B->s1.ensure(bytes);
+        B->s2T.ensure(bytes);
+        xt = B->xt.as<T>(), xtT = B->xtT.as<T>(), s1 = B->s1.as<T>(), s2T = B->s2T.as<T>();
+    }
     dim3 g2((unsigned)((NB + 31) / 32), (unsigned)((NA + 31) / 32));
     HIPOTF_REQUIRE(g2.y < 65536, "too many alpha strings for the tiled kernels");
     hipLaunchKernelGGL((k_gauge_in<T, NV>), g2, dim3(256), 0, st, src, xt, xtT, Bs.strs.as<u64>(),
@@ -886,6 +900,46 @@ void sigma(DevHam *H, int nv, const double *const *src, double *const *dst, int 
     HIPOTF_REQUIRE(nv >= 1, "need at least one vector");
     if (prec == 0) sigma_t<double>(H, nv, src, dst, st);
     else sigma_t<float>(H, nv, src, dst, st);
+}
+
+// ---------------------------------------------------------------------------------------
+// allocation-free sigma: the work buffers live in a static device array (64 MiB per GPU, held
+// until the process ends); host code reaches it only through the address hipGetSymbolAddress
+// returns for the current device, looked up once per device on first use
+// ---------------------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+static unsigned char *scratch_address() {
+    static std::mutex mu;
+    static void *addr[256] = {};
+    int dev = 0;
+    HIPOTF_CHECK(hipGetDevice(&dev));
+    HIPOTF_REQUIRE(dev >= 0 && dev < 256, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        HIPOTF_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        HIPOTF_REQUIRE(p != nullptr, "scratch symbol has no device address");
+        addr[dev] = p;
+    }
+    return static_cast<unsigned char *>(addr[dev]);
+}
+
+void sigma_noalloc(DevHam *H, const double *src, double *dst, hipStream_t st) {
+    HIPOTF_REQUIRE(H != nullptr && src && dst, "NULL argument");
+    const i64 dim = H->basis->host.dim;
+    HIPOTF_REQUIRE(dim <= NOALLOC_MAX_DIM, "allocation-free sigma: " + std::to_string(dim) +
+                                               " determinants exceed its scratch limit of " +
+                                               std::to_string(NOALLOC_MAX_DIM));
+    const size_t stride = ((size_t)dim * sizeof(double) + 255) / 256 * 256;
+    HIPOTF_REQUIRE(4 * stride <= NOALLOC_SCRATCH_BYTES, "internal: scratch layout exceeds the array");
+    unsigned char *work = scratch_address();
+    ForbidAlloc guard;
+    SrcPack sp{};
+    DstPack dp{};
+    sp.p[0] = src;
+    dp.p[0] = dst;
+    sigma_group<double, 1>(H, sp, dp, st, work);
 }
 
 void diag_elements(DevHam *H, double *diags, hipStream_t st) {

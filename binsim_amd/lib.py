@@ -50,6 +50,7 @@ SIGNATURES = {
     "hipotf_set_tuning": (C.c_int, [C.c_int, _i32p, C.c_int]),
     "hipotf_get_tuning": (C.c_int, [C.c_int, _i32p, C.c_int]),
     "hipotf_hvec_block": (C.c_int, [_vp, _vp, C.c_int, _dpp, _dpp, C.c_int]),
+    "hipotf_hvec_noalloc": (C.c_int, [_vp, _vp, _vp, _vp]),
     "hipotf_vqe_energy_grad": (C.c_int, [_vp, _vp, _vp, C.c_int64, _i64p, _dp, _vp, _vp, _dp, _dp]),
     "hipotf_vqe_energy": (C.c_int, [_vp, _vp, _vp, C.c_int64, _i64p, _dp, _vp, _vp, _dp]),
     "hipotf_davidson_f64": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,

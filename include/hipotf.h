@@ -111,6 +111,16 @@ int hipotf_prepare(void *basis, void *net);
  * choice as JSON.  Return 0 on success. */
 int hipotf_set_tuning(int precision, const int32_t *v, int n);
 int hipotf_get_tuning(int precision, int32_t *v, int n);
+/* hvec_cuda without device allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  dst = H src in FP64 as hvec_cuda computes it, with the four sigma
+ * work buffers in the library's static device scratch array (64 MiB per GPU, held until the
+ * process ends), so at most 2,097,152 determinants (larger bases are refused).  The operator's
+ * device tables must already exist for this basis (hipotf_prepare or one hvec_cuda call);
+ * otherwise the call is refused rather than building them.  Calls on one GPU share the
+ * scratch array: calls that overlap on one GPU (several host threads or streams) race.
+ * Returns 0 on success, 1 on error (message in hipotf_last_error()). */
+int hipotf_hvec_noalloc(void *basis, void *net, const double *src, double *dst);
 /* block sigma: dst[v] = H src[v], v < nvec (host arrays of device pointers).
  * precision 0: float64 arithmetic (as hvec_cuda); 1: float32 coefficients and integrals with
  * float64 input/output vectors (the mixed-precision Davidson bulk). */
