@@ -177,75 +177,6 @@ static void decode_hamiltonian(const HostNet &N, DevNet &D) {
 }
 
 // ---- checking a decode against the network it came from ------------------------------------
-// Determinants as interleaved spin-orbital occupations: bit 2p = alpha p, bit 2p+1 = beta p
-// (src/include/ham/ham.hpp:519-551).
-static u64 interleave(u32 a, u32 b) {
-    u64 x = 0;
-    for (int p = 0; p < 32; ++p) x |= (u64)((a >> p) & 1u) << (2 * p) | (u64)((b >> p) & 1u) << (2 * p + 1);
-    return x;
-}
-
-// apply one ladder operator at spin-orbital pos to occupation x (Jordan-Wigner sign: parity of
-// the occupied spin-orbitals below pos); false if it annihilates the state
-static bool ladder_apply(u64 &x, int &par, int pos, bool create) {
-    const u64 bit = 1ull << pos;
-    if (((x & bit) != 0) == create) return false;
-    par += __builtin_popcountll(x & (bit - 1));
-    x ^= bit;
-    return true;
-}
-
-// <bra| H |ket> of the two-body operator held by D (ecore, h_pq, (pq|rs) chemist order), by the
-// Slater-Condon rules in the interleaved spin-orbital order, with the signs of the ladder
-// operators applied to ket:  H = ecore + sum h_PQ a+_P a_Q + 1/2 sum (PQ|RS) a+_P a+_R a_S a_Q
-static double sc_element(const DevNet &D, u64 bra, u64 ket) {
-    const int n = D.n;
-    auto h1 = [&](int P, int Q) { return ((P ^ Q) & 1) ? 0.0 : D.h[(size_t)(P >> 1) * n + (Q >> 1)]; };
-    auto g2 = [&](int P, int Q, int R, int S) {
-        if (((P ^ Q) & 1) || ((R ^ S) & 1)) return 0.0;
-        return D.eri[((((size_t)(P >> 1) * n + (Q >> 1)) * n + (R >> 1)) * n) + (S >> 1)];
-    };
-    const u64 diff = bra ^ ket;
-    const int nd = __builtin_popcountll(diff);
-    std::vector<int> occ;
-    for (u64 m = ket; m; m &= m - 1) occ.push_back(__builtin_ctzll(m));
-    if (nd == 0) {
-        double e = D.ecore;
-        for (int P : occ) e += h1(P, P);
-        for (int P : occ)
-            for (int R : occ)
-                if (P != R) e += 0.5 * (g2(P, P, R, R) - g2(P, R, R, P));
-        return e;
-    }
-    if (nd == 2) {
-        const int Q = __builtin_ctzll(ket & diff), P = __builtin_ctzll(bra & diff);
-        if (!(ket & diff) || !(bra & diff)) return 0.0;
-        u64 x = ket;
-        int par = 0;
-        ladder_apply(x, par, Q, false);
-        ladder_apply(x, par, P, true);
-        double v = h1(P, Q);
-        for (int R : occ)
-            if (R != Q) v += g2(P, Q, R, R) - g2(P, R, R, Q);
-        return (par & 1) ? -v : v;
-    }
-    if (nd == 4) {
-        const u64 an = ket & diff, cr = bra & diff;
-        if (__builtin_popcountll(an) != 2 || __builtin_popcountll(cr) != 2) return 0.0;
-        const int Q1 = __builtin_ctzll(an), Q2 = 63 - __builtin_clzll(an);
-        const int P1 = __builtin_ctzll(cr), P2 = 63 - __builtin_clzll(cr);
-        u64 x = ket;
-        int par = 0;
-        ladder_apply(x, par, Q1, false);
-        ladder_apply(x, par, Q2, false);
-        ladder_apply(x, par, P2, true);
-        ladder_apply(x, par, P1, true);
-        const double v = g2(P1, Q1, P2, Q2) - g2(P1, Q2, P2, Q1);
-        return (par & 1) ? -v : v;
-    }
-    return 0.0;
-}
-
 // deterministic pseudo-random subset of the bits of m (splitmix64 of the key)
 static u32 probe_subset(u32 m, u64 key) {
     u64 z = key + 0x9E3779B97F4A7C15ull;
@@ -299,7 +230,7 @@ static void verify_hamiltonian(const HostNet &N, const DevNet &D) {
     u64 nprobe = 0;
     auto check = [&](u32 ax, u32 bx, u32 a, u32 b) {
         const double fn = N.element(ax, bx, a, b);
-        const double fs = sc_element(D, interleave(a ^ ax, b ^ bx), interleave(a, b));
+        const double fs = slater_condon(D.n, D.h.data(), D.eri.data(), D.ecore, interleave_occ(a ^ ax, b ^ bx), interleave_occ(a, b));
         const double d = std::fabs(fn - fs);
         ++nprobe;
         if (!(d <= tol)) {
@@ -473,7 +404,7 @@ static void verify_pool(const HostNet &N, const DevNet &D) {
                     const u32 sa = r == 0 ? 0u : r == 1 ? fa : probe_subset(fa, key);
                     const u32 sb = r == 0 ? 0u : r == 1 ? fb : probe_subset(fb, key ^ 0x5Aull);
                     const u32 a = anna | sa, b = annb | sb;
-                    const u64 ket = interleave(a, b), bra = interleave(a ^ G.ax, b ^ G.bx);
+                    const u64 ket = interleave_occ(a, b), bra = interleave_occ(a ^ G.ax, b ^ G.bx);
                     u64 o = 0;
                     double e = 0.0;
                     int s = apply_term(T, ket, o);

@@ -37,6 +37,10 @@ void block_dots(i64 N, int na, const double *const *A, int nbv, const double *co
 void block_subtract(i64 N, int m, const double *const *V, const double *P, int nt, double *const *T,
                     hipStream_t st);
 void rotate_inplace(i64 N, int m, int keep, const double *C, double *const *X, hipStream_t st);
+void scatter_values(double *v, const i64 *idx, const double *val, int count, hipStream_t st);
+
+// determinants of the P space of the initial guess (lowest diagonals)
+constexpr int PSPACE = 256;
 
 // eigen-decomposition of a symmetric m x m matrix (row-major a), ascending eigenvalues,
 // eigenvectors in the columns of v (row-major)
@@ -50,11 +54,16 @@ void jacobi_eigh(int m, std::vector<double> a, std::vector<double> &w, std::vect
                 tot += a[i * m + j] * a[i * m + j];
                 if (i != j) off += a[i * m + j] * a[i * m + j];
             }
-        if (off <= 1e-32 * tot || off == 0.0) break;
+        // converged when the off-diagonal part is at rounding level (|off| <= 1e-15 |A|), or
+        // when a sweep finds nothing above it to rotate away
+        if (off <= 1e-30 * tot || off == 0.0) break;
+        const double small = 1e-17 * std::sqrt(tot);
+        int rotated = 0;
         for (int p = 0; p < m - 1; ++p)
             for (int q = p + 1; q < m; ++q) {
                 const double apq = a[p * m + q];
-                if (std::fabs(apq) < 1e-300) continue;
+                if (std::fabs(apq) <= small) continue;
+                ++rotated;
                 const double app = a[p * m + p], aqq = a[q * m + q];
                 const double theta = (aqq - app) / (2.0 * apq);
                 const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
@@ -75,6 +84,7 @@ void jacobi_eigh(int m, std::vector<double> a, std::vector<double> &w, std::vect
                     v[k * m + q] = s * vkp + c * vkq;
                 }
             }
+        if (rotated == 0) break;
     }
     std::vector<int> order(m);
     std::iota(order.begin(), order.end(), 0);
@@ -153,18 +163,40 @@ DavidsonOut davidson(DevHam *H, int nroots, int mode, int block, double tol, dou
         AV[i] = AVb[i].as<double>();
     }
 
-    // initial subspace: the caller's nroots vectors, else unit vectors on the lowest diagonals;
-    // the block mode tops it up to k vectors with further lowest-diagonal unit vectors
+    // initial subspace: the caller's nroots vectors, else the lowest eigenvectors of H in the
+    // "P space" of the NP lowest-diagonal determinants (H_PP built on the host by Slater-Condon
+    // rules, the FCI codes' usual start), else unit vectors on the lowest diagonals; the block
+    // mode tops it up to k vectors
     std::vector<i64> low;
     {
         std::vector<double> dh(N);
         HIPOTF_CHECK(hipMemcpy(dh.data(), diag.p, N * sizeof(double), hipMemcpyDeviceToHost));
         std::vector<i64> idx(N);
         std::iota(idx.begin(), idx.end(), 0);
-        const i64 nl = std::min<i64>(N, (i64)k + nroots);
+        const i64 nl = std::min<i64>(N, std::max<i64>((i64)k + nroots, PSPACE));
         std::partial_sort(idx.begin(), idx.begin() + nl, idx.end(),
                           [&](i64 a, i64 b) { return dh[a] < dh[b] || (dh[a] == dh[b] && a < b); });
         low.assign(idx.begin(), idx.begin() + nl);
+    }
+    std::vector<std::vector<double>> pvec;  // P-space eigenvectors over low[0..np)
+    const int np = (int)std::min<i64>(PSPACE, (i64)low.size());
+    if (!guess && np >= std::max(k, nroots) + 1 && H->basis->host.norb <= 32) {
+        const HostBasis &hb = H->basis->host;
+        const i64 NB = hb.B.N;
+        std::vector<u64> occ(np);
+        for (int i = 0; i < np; ++i)
+            occ[i] = interleave_occ((u32)hb.A.strs[low[i] / NB], (u32)hb.B.strs[low[i] % NB]);
+        std::vector<double> hp((size_t)np * np), wp, cp;
+        for (int i = 0; i < np; ++i)
+            for (int j = 0; j <= i; ++j)
+                hp[(size_t)i * np + j] = hp[(size_t)j * np + i] =
+                    slater_condon(H->n, H->h.data(), H->eri.data(), H->ecore, occ[i], occ[j]);
+        jacobi_eigh(np, hp, wp, cp);
+        const int nv0 = mode == 1 ? k : nroots;
+        for (int i = 0; i < nv0; ++i) {
+            pvec.emplace_back(np);
+            for (int j = 0; j < np; ++j) pvec.back()[j] = cp[(size_t)j * np + i];
+        }
     }
     const double one = 1.0;
     int m = 0;
@@ -185,7 +217,16 @@ DavidsonOut davidson(DevHam *H, int nroots, int mode, int block, double tol, dou
         ++m;
     };
     size_t next_low = 0;
-    for (int i = 0; i < nroots; ++i) {
+    if (!pvec.empty()) {
+        std::vector<i64> pidx(low.begin(), low.begin() + np);
+        for (const auto &pv : pvec) {
+            HIPOTF_CHECK(hipMemsetAsync(V[m], 0, N * sizeof(double), st));
+            scatter_values(V[m], pidx.data(), pv.data(), np, st);
+            accept(true);
+        }
+        next_low = low.size();  // nothing to top up with
+    }
+    for (int i = 0; i < nroots && pvec.empty(); ++i) {
         if (guess)
             HIPOTF_CHECK(hipMemcpyAsync(V[m], guess + (size_t)i * N, N * sizeof(double),
                                         hipMemcpyDeviceToDevice, st));

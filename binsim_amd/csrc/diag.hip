@@ -547,6 +547,24 @@ void rotate_inplace(i64 N, int m, int keep, const double *C, double *const *X, h
     HIPOTF_CHECK(hipStreamSynchronize(st));
 }
 
+__global__ __launch_bounds__(256) void k_scatter(double *__restrict__ v, const i64 *__restrict__ idx,
+                                                 const double *__restrict__ val, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) v[idx[i]] = val[i];
+}
+
+// v[idx[i]] = val[i], i < count (host arrays)
+void scatter_values(double *v, const i64 *idx, const double *val, int count, hipStream_t st) {
+    if (count <= 0) return;
+    DevBuf di((size_t)count * sizeof(i64)), dv((size_t)count * sizeof(double));
+    HIPOTF_CHECK(hipMemcpyAsync(di.p, idx, (size_t)count * sizeof(i64), hipMemcpyHostToDevice, st));
+    HIPOTF_CHECK(hipMemcpyAsync(dv.p, val, (size_t)count * sizeof(double), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_scatter, dim3((count + 255) / 256), dim3(256), 0, st, v, di.as<i64>(), dv.as<double>(),
+                       count);
+    HIPOTF_CHECK(hipGetLastError());
+    HIPOTF_CHECK(hipStreamSynchronize(st));
+}
+
 // G[a][b] = A_a . B_b into host row-major [na][nb] (na <= DMAX, nb <= BK)
 void block_dots(i64 N, int na, const double *const *A, int nbv, const double *const *B, double *G,
                 hipStream_t st) {

@@ -188,6 +188,75 @@ const RowBlockTable &row_blocks(SameSpinTable &T, int rb) {
     return *T.blocks[slot];
 }
 
+// Determinants as interleaved spin-orbital occupations: bit 2p = alpha p, bit 2p+1 = beta p
+// (src/include/ham/ham.hpp:519-551).
+u64 interleave_occ(u32 a, u32 b) {
+    u64 x = 0;
+    for (int p = 0; p < 32; ++p) x |= (u64)((a >> p) & 1u) << (2 * p) | (u64)((b >> p) & 1u) << (2 * p + 1);
+    return x;
+}
+
+// apply one ladder operator at spin-orbital pos to occupation x (Jordan-Wigner sign: parity of
+// the occupied spin-orbitals below pos); false if it annihilates the state
+bool ladder_apply(u64 &x, int &par, int pos, bool create) {
+    const u64 bit = 1ull << pos;
+    if (((x & bit) != 0) == create) return false;
+    par += __builtin_popcountll(x & (bit - 1));
+    x ^= bit;
+    return true;
+}
+
+// <bra| H |ket> of the two-body operator (ecore, h[p*n+q], eri[((p*n+q)*n+r)*n+s] = (pq|rs)
+// chemist order), by the Slater-Condon rules in the interleaved spin-orbital order, with the
+// signs of the ladder operators applied to ket:
+//   H = ecore + sum h_PQ a+_P a_Q + 1/2 sum (PQ|RS) a+_P a+_R a_S a_Q
+double slater_condon(int n, const double *hh, const double *eri, double ecore, u64 bra, u64 ket) {
+    auto h1 = [&](int P, int Q) { return ((P ^ Q) & 1) ? 0.0 : hh[(size_t)(P >> 1) * n + (Q >> 1)]; };
+    auto g2 = [&](int P, int Q, int R, int S) {
+        if (((P ^ Q) & 1) || ((R ^ S) & 1)) return 0.0;
+        return eri[((((size_t)(P >> 1) * n + (Q >> 1)) * n + (R >> 1)) * n) + (S >> 1)];
+    };
+    const u64 diff = bra ^ ket;
+    const int nd = __builtin_popcountll(diff);
+    std::vector<int> occ;
+    for (u64 m = ket; m; m &= m - 1) occ.push_back(__builtin_ctzll(m));
+    if (nd == 0) {
+        double e = ecore;
+        for (int P : occ) e += h1(P, P);
+        for (int P : occ)
+            for (int R : occ)
+                if (P != R) e += 0.5 * (g2(P, P, R, R) - g2(P, R, R, P));
+        return e;
+    }
+    if (nd == 2) {
+        const int Q = __builtin_ctzll(ket & diff), P = __builtin_ctzll(bra & diff);
+        if (!(ket & diff) || !(bra & diff)) return 0.0;
+        u64 x = ket;
+        int par = 0;
+        ladder_apply(x, par, Q, false);
+        ladder_apply(x, par, P, true);
+        double v = h1(P, Q);
+        for (int R : occ)
+            if (R != Q) v += g2(P, Q, R, R) - g2(P, R, R, Q);
+        return (par & 1) ? -v : v;
+    }
+    if (nd == 4) {
+        const u64 an = ket & diff, cr = bra & diff;
+        if (__builtin_popcountll(an) != 2 || __builtin_popcountll(cr) != 2) return 0.0;
+        const int Q1 = __builtin_ctzll(an), Q2 = 63 - __builtin_clzll(an);
+        const int P1 = __builtin_ctzll(cr), P2 = 63 - __builtin_clzll(cr);
+        u64 x = ket;
+        int par = 0;
+        ladder_apply(x, par, Q1, false);
+        ladder_apply(x, par, Q2, false);
+        ladder_apply(x, par, P2, true);
+        ladder_apply(x, par, P1, true);
+        const double v = g2(P1, Q1, P2, Q2) - g2(P1, Q2, P2, Q1);
+        return (par & 1) ? -v : v;
+    }
+    return 0.0;
+}
+
 void integrals_from_reference_layout(int n, const double *h_cm, const double *g_ref,
                                      std::vector<double> &h, std::vector<double> &eri) {
     HIPOTF_REQUIRE(n >= 1 && n <= 32, "norb must be in [1, 32]");

@@ -198,6 +198,13 @@ struct DevHam {
     SameSpinTable *ptA = nullptr, *ptB = nullptr;
 };
 
+// determinants as interleaved spin-orbital occupations (bit 2p = alpha p, 2p+1 = beta p)
+u64 interleave_occ(u32 a, u32 b);
+// one ladder operator on occupation x; adds the Jordan-Wigner parity to par; false if zero
+bool ladder_apply(u64 &x, int &par, int pos, bool create);
+// <bra|H|ket> of the two-body Hamiltonian (h, (pq|rs), ecore) by Slater-Condon rules with
+// interleaved Jordan-Wigner signs (the reference's convention)
+double slater_condon(int n, const double *h, const double *eri, double ecore, u64 bra, u64 ket);
 void integrals_from_reference_layout(int n, const double *h_colmajor, const double *g_ref_colmajor,
                                      std::vector<double> &h, std::vector<double> &eri);
 DevHam *make_dev_ham(DevBasis *b, int n, const std::vector<double> &h, const std::vector<double> &eri,
