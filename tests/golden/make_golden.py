@@ -7,7 +7,8 @@ oracle/glue.py and oracle/oracle.py drive them; the Pauli terms come from the re
 generate_hamiltonian_real_64_128_f64.
 
     python tests/golden/make_golden.py            # all fixtures (~15 minutes on 8 cores)
-    python tests/golden/make_golden.py config1    # one set: small | config0 | config1
+    python tests/golden/make_golden.py config1    # one set: small | config0 | config1 | config3
+                                                  # (config3: 18 orbitals, ~25 minutes, ~40 GB RAM)
 """
 from __future__ import annotations
 
@@ -145,6 +146,41 @@ def config1_fixtures(R):
     print("config1", meta, flush=True)
 
 
+def config3_fixtures(R):
+    """18 orbitals, 7+7 electrons (1,012,766,976 determinants, 8.1 GB vectors): one reference sigma
+    of the seeded benchmark vector (seed 1000 + 18), kept as checksums and 4096 sampled entries,
+    and the diagonal's head and samples.  Vectors are normalised in place to bound host memory."""
+    n, na, nb = 18, 7, 7
+    h, eri = for_norb(n)
+    h1, g2 = one_body_layout(h), reference_layout(eri)
+    t0 = time.time()
+    S = ref_system(R, n, na, nb, h1, g2)
+    dim = S["dim"]
+    x = np.random.default_rng(1000 + 18).uniform(-1.0, 1.0, dim)
+    x /= np.linalg.norm(x)
+    t1 = time.time()
+    print("config3 network", t1 - t0, "s", flush=True)
+    sigma = R.hvec(S["basis"], S["ham"], x)
+    t_sigma = time.time() - t1
+    print("config3 sigma", t_sigma, "s", flush=True)
+    del x
+    samp = np.sort(np.random.default_rng(99).choice(dim, 4096, replace=False))
+    out = dict(sigma_sum=sigma.sum(), sigma_norm2=sigma @ sigma, sample_idx=samp, sample_val=sigma[samp])
+    y = np.random.default_rng(4242).uniform(-1.0, 1.0, dim)
+    y /= np.linalg.norm(y)
+    out["sigma_dot_y"] = y @ sigma
+    del y, sigma
+    diag = R.diags(S["basis"], S["ham"], dim)
+    out.update(diag_head=diag[:256], diag_sum=diag.sum(), diag_sample=diag[samp], nterms=S["nterms"])
+    del diag
+    np.savez_compressed(os.path.join(HERE, "config3_n18.npz"), **out)
+    meta = dict(integrals_sha256=digest(h, eri), build_s=t1 - t0, sigma_s=t_sigma,
+                threads=os.environ.get("OMP_NUM_THREADS", "default"))
+    with open(os.path.join(HERE, "config3_timing.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    print("config3", meta, flush=True)
+
+
 if __name__ == "__main__":
     R = Reference()
     which = sys.argv[1:] or ["small", "config0", "config1"]
@@ -154,3 +190,5 @@ if __name__ == "__main__":
         config0_fixtures(R)
     if "config1" in which:
         config1_fixtures(R)
+    if "config3" in which:
+        config3_fixtures(R)

@@ -1,9 +1,11 @@
 """The sigma build at the benchmark shapes themselves, checked through what holds at any size.
 
-* 16 orbitals, 8+8 electrons (165,636,900 determinants, the benchmark workload): the FP64
-  sigma of the seeded benchmark vector against the reference CPU library's sigma of the same
-  vector (tests/golden/config1_n16.npz: sum, squared norm, dot with a second seeded vector,
-  4096 sampled entries; the diagonal's head and the same samples), within 1e-12 relative.
+* 16 orbitals, 8+8 electrons (165,636,900 determinants, the benchmark workload) and 18
+  orbitals, 7+7 electrons (1,012,766,976 determinants): the FP64 sigma of the seeded benchmark
+  vector against the reference CPU library's sigma of the same vector
+  (tests/golden/config1_n16.npz, config3_n18.npz: sum, squared norm, dot with a second seeded
+  vector, 4096 sampled entries; the diagonal's head and the same samples), within 1e-12
+  relative.
 * 16 orbitals and 18 orbitals / 7+7 electrons (1,012,766,976 determinants, 8.1 GB vectors):
   Hermiticity <y|Hx> = <Hy|x> (1e-12 relative), the closed-shell HF energy
   2 sum_i h_ii + sum_ij [2 (ii|jj) - (ij|ji)] at index 0 of the diagonal and of H e_0 (1e-12),
@@ -81,13 +83,14 @@ def _properties(torch, n, k, det_check):
             AT.set_config(AT.FP64, saved)
 
 
-def test_sigma_16_orbitals_matches_reference(torch_cuda):
+@pytest.mark.parametrize("n,k,fixture", [(16, 8, "config1_n16.npz"), (18, 7, "config3_n18.npz")])
+def test_sigma_full_size_matches_reference(torch_cuda, n, k, fixture):
     torch = torch_cuda
-    print("full-size reference sigma n=16 k=8", flush=True)
-    f = np.load(os.path.join(GOLDEN, "config1_n16.npz"))
-    _, _, B, H, F = _system(16, 8)
+    print(f"full-size reference sigma n={n} k={k}", flush=True)
+    f = np.load(os.path.join(GOLDEN, fixture))
+    _, _, B, H, F = _system(n, k)
     dim = B.dim
-    x = _seeded_host(torch, dim, 1000 + 16)
+    x = _seeded_host(torch, dim, 1000 + n)
     s = torch.empty_like(x)
     F.hvec(x, s)
     del x
@@ -106,6 +109,8 @@ def test_sigma_16_orbitals_matches_reference(torch_cuda):
     dmax = float(np.max(np.abs(f["diag_head"])))
     assert np.max(np.abs(d[:256].cpu().numpy() - f["diag_head"])) <= 1e-12 * dmax
     assert np.max(np.abs(d[idx].cpu().numpy() - f["diag_sample"])) <= 1e-12 * dmax
+    del s, d
+    torch.cuda.empty_cache()
 
 
 def test_sigma_16_orbitals_properties(torch_cuda):
