@@ -7,21 +7,24 @@ binsim_amd/integrals.py).  A step is one FP64 sigma build H x (hvec_cuda) on a r
 seeded vector; ``value`` is its algorithmic HBM rate  B_sigma / t  with
 B_sigma = 8 * dim * (2 + s_a + d_a) bytes (s_a = 64 singles, d_a = 784 doubles per string).
 
-Reported beside it (one run each, outside the timed steps):
-  * FCI, lowest 4 roots: plain FP64 Davidson (one FP64 sigma per vector) and mixed-precision
-    block Davidson (FP32 block sigma, FP64 Rayleigh-Ritz, FP64 refinement), wall seconds and
-    per-root |dE| -- at 12 orbitals / 6+6 electrons to residual 1e-8 by default, and at the
-    16-orbital workload to residual 1e-6 with --full (each solve takes minutes there);
-  * FCI at 12 orbitals / 6+6 electrons, one root from HF to 1e-8 (the CPU-runnable case);
-  * VQE energy + adjoint gradient (fused) at 12 orbitals (1,818 parameters) and at 16 orbitals
-    (5,792 parameters): gradients/s and max deviation from central differences (h = 1e-6);
-  * cpu_baseline: the reference's own CPU sigma (oracle/_ref/libotf_native, OpenMP) on a
-    bounded sample, all host cores.
+Reported beside it (one run each, outside the timed steps, in this order; --budget seconds
+bound the whole run, 500 by default):
+  * FCI, lowest 4 roots at the 16-orbital workload to residual 1e-6: plain FP64 Davidson (one
+    FP64 sigma per correction vector) and mixed-precision block Davidson (k = 4: FP32 block
+    sigma, FP64 Rayleigh-Ritz, FP64 refinement), wall seconds, sigma counts and per-root |dE|;
+  * VQE energy + adjoint gradient (fused) at 16 orbitals (5,792 parameters): gradients/s and the
+    max deviation from central differences (h = 1e-6) on 64 sampled parameters (as many as the
+    budget allows; fd_params says how many ran);
+  * FCI at 12 orbitals / 6+6 electrons, one root from HF to 1e-8 (the CPU-runnable case), and
+    the VQE energy + gradient there (1,818 parameters);
+  * cpu_baseline: the reference's own CPU sigma (oracle/_ref/libotf_native, OpenMP) at 12
+    orbitals on a bounded sample, all host cores, and its Davidson and VQE evaluation there.
 
-    python bench.py [--steps K] [--warmup W] [--quick]
+    python bench.py [--steps K] [--warmup W] [--quick] [--no-fci16] [--fci12]
     python bench.py --davidson-only 600     # 16-orbital ground state, plain vs mixed
     python bench.py --sigma-shape 18,7      # FP64 / FP32 sigma at another shape
     python bench.py --vqe-only 16           # VQE energy + gradient only
+    python bench.py --cpu-sigma 16          # the reference CPU sigma at 16 orbitals only
 """
 from __future__ import annotations
 
@@ -89,20 +92,27 @@ def timed_sigma(torch, F, x, y, steps, warmup):
     return t0.elapsed_time(t1) / steps, wall
 
 
-def fci_block(torch, B, H, nroots, maxspace, tol, maxiter):
+def fci_block(torch, B, H, nroots, maxspace, tol, maxiter, maxiter_mixed=None):
+    """Plain FP64 Davidson (one FP64 sigma per correction vector) vs mixed-precision block Davidson
+    (k = nroots, FP32 block sigma, FP64 Rayleigh-Ritz / residuals, FP64 refinement), both from the
+    same P-space start; per-root |dE| and the wall-time ratio."""
     from binsim_amd import davidson as D
-    out = {"tol": tol, "maxspace": maxspace}
-    for name, mixed in (("plain_fp64", False), ("mixed_block", True)):
-        e, _, info = D.solve(B, H, nroots=nroots, mixed=mixed, tol=tol, maxiter=maxiter,
+    out = {"nroots": nroots, "tol": tol, "maxspace": maxspace, "block": nroots,
+           "start": "lowest eigenvectors of H in the 256 lowest-diagonal determinants (P space)"}
+    for name, mixed, mi in (("plain_fp64", False, maxiter), ("mixed_block", True, maxiter_mixed or maxiter)):
+        e, _, info = D.solve(B, H, nroots=nroots, mixed=mixed, tol=tol, maxiter=mi, block=nroots,
                              maxspace=maxspace, verbose=True)
         out[name] = dict(seconds=round(info["seconds"], 3), iterations=int(info["iterations"]),
-                         sigma_fp64=int(info["sigma64"]), sigma_fp32=int(info["sigma32"]),
+                         maxiter=int(mi), sigma_fp64=int(info["sigma64"]), sigma_fp32=int(info["sigma32"]),
                          converged=bool(info["converged"] == 1),
                          max_residual=info["max_residual"], energies=[float(v) for v in e])
+        if mixed:
+            out[name]["fp64_refinement_from_iteration"] = int(info["switch_iter"])
         log(f"fci {name}: {info['seconds']:.2f} s, {int(info['iterations'])} it, e = {e}")
     import numpy as np
     de = np.abs(np.array(out["plain_fp64"]["energies"]) - np.array(out["mixed_block"]["energies"]))
     out["abs_dE_per_root"] = [float(v) for v in de]
+    out["max_abs_dE"] = float(de.max())
     out["speedup_mixed_vs_plain"] = round(out["plain_fp64"]["seconds"] / out["mixed_block"]["seconds"], 3)
     return out
 
@@ -156,8 +166,13 @@ def fci16_ground(torch, maxiter):
     return out
 
 
-def vqe(torch, n, na, nb, nfd, seed_off=2000, amp=0.05, stored_tuning=False):
+def vqe(torch, n, na, nb, nfd, seed_off=2000, amp=0.05, stored_tuning=False, fd_budget_s=None):
+    """Fused energy + adjoint gradient, then central differences (h = 1e-6) of the energy-only
+    entry point on nfd parameters sampled with seed + 3000 (binsim_amd.vqe.fd_gradient: a running
+    prefix state, so each evaluation sweeps only the generators from the sampled one on).  With
+    fd_budget_s the sample is cut short when the time runs out (fd_params says how many ran)."""
     import numpy as np
+    from binsim_amd.vqe import fd_gradient
     B, H, P, F = build(n, na, nb, pool=True)
     if stored_tuning:
         from binsim_amd import autotune as AT
@@ -169,12 +184,13 @@ def vqe(torch, n, na, nb, nfd, seed_off=2000, amp=0.05, stored_tuning=False):
     npar = len(ucc_spin_orbitals(n, na, nb))
     th = np.random.default_rng(SEED_BASE + seed_off).uniform(-amp, amp, npar)
     idx = np.arange(npar)
-    psi = torch.zeros(B.dim, dtype=torch.float64, device="cuda")
+    hf = torch.zeros(B.dim, dtype=torch.float64, device="cuda")
+    hf[0] = 1.0
+    psi = hf.clone()
     lam = torch.empty_like(psi)
 
     def run(x):
-        psi.zero_()
-        psi[0] = 1.0
+        psi.copy_(hf)
         return F.vqe_energy_grad(idx, x, psi, lam)
 
     run(th)
@@ -185,24 +201,32 @@ def vqe(torch, n, na, nb, nfd, seed_off=2000, amp=0.05, stored_tuning=False):
         e, g, d2, _ = run(th)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t) / reps
+    back_to_hf = float((psi - hf).abs().max())
     rng = np.random.default_rng(SEED_BASE + 3000)
     sample = np.sort(rng.choice(npar, min(nfd, npar), replace=False))
-    h = 1e-6
-    dev = 0.0
-    for k in sample:
-        xp, xm = th.copy(), th.copy()
-        xp[k] += h
-        xm[k] -= h
-        fd = (run(xp)[0] - run(xm)[0]) / (2 * h)
-        dev = max(dev, abs(fd - g[k]))
-    log(f"vqe n={n}: {dt * 1e3:.1f} ms per energy+grad, fd dev {dev:.2e} over {len(sample)}")
+    work = [torch.empty_like(psi) for _ in range(3)]
+    done, fd = [], []
+    t_fd = time.perf_counter()
+    # in chunks of 8 (sorted positions), so the budget check runs between chunks
+    for c0 in range(0, len(sample), 8):
+        if fd_budget_s is not None and time.perf_counter() - t_fd > fd_budget_s:
+            break
+        chunk = sample[c0:c0 + 8]
+        fd.extend(fd_gradient(F, idx, th, chunk, 1e-6, hf, work).tolist())
+        done.extend(chunk.tolist())
+    t_fd = time.perf_counter() - t_fd
+    dev = float(np.max(np.abs(np.array(fd) - g[done]))) if done else None
+    log(f"vqe n={n}: {dt * 1e3:.1f} ms per energy+grad, fd dev {dev} over {len(done)} ({t_fd:.1f} s)")
+    del work
     return dict(parameters=npar, seconds_per_eval=round(dt, 5), grads_per_s=round(1.0 / dt, 3),
-                energy=e, dH2=d2, fd_max_dev=dev, fd_params=int(len(sample)))
+                energy=e, dH2=d2, fd_max_dev=dev, fd_params=len(done), fd_seconds=round(t_fd, 2),
+                fd_h=1e-6, fd_sample_seed=SEED_BASE + 3000, state_back_to_hf_max_dev=back_to_hf)
 
 
-def cpu_baseline(budget_s=30.0, dav_iters=25):
+def cpu_baseline(budget_s=10.0, extras=True, dav_iters=200):
     """The reference's OpenMP CPU path (oracle/_ref) on the 12-orbital system: the sigma,
-    repeated (the reported value), then a bounded Davidson sample and one VQE evaluation."""
+    repeated (the reported value), then (extras) the Davidson ground state from HF to 1e-8 and
+    one VQE energy + gradient."""
     import numpy as np
     from oracle import glue
     from oracle.ref import Reference, available
@@ -232,9 +256,11 @@ def cpu_baseline(budget_s=30.0, dav_iters=25):
                       f"on 12 orbitals / 6+6 electrons ({dim} dets), {reps} calls, "
                       f"{dt * 1e3:.1f} ms per sigma, algorithmic bytes of that shape",
                seconds_per_sigma=dt)
-    # the same system's Davidson from HF (jl/davidson.jl over the reference libdiag and sigma;
-    # 80 iterations to 1e-8, here the first dav_iters of them) and one VQE energy + gradient
-    # (jl/vqe.jl over the reference expm / sigma / backgrad, all 1,818 parameters)
+    if not extras:
+        return out
+    # the same system's Davidson from HF to 1e-8 (jl/davidson.jl over the reference libdiag and
+    # sigma) and one VQE energy + gradient (jl/vqe.jl over the reference expm / sigma /
+    # backgrad, all 1,818 parameters)
     diag = R.diags(bp, net, dim)
     hf = np.zeros(dim)
     hf[0] = 1.0
@@ -252,11 +278,41 @@ def cpu_baseline(budget_s=30.0, dav_iters=25):
     t_vqe = time.perf_counter() - t
     out["other"] = {
         "davidson_12o_from_hf": dict(iterations=its, converged=conv, seconds=round(t_dav, 3),
-                                     s_per_iteration=round(t_dav / its, 4), energy=e0,
-                                     note="first iterations of the 80 the solve takes to 1e-8"),
+                                     s_per_iteration=round(t_dav / its, 4), energy=e0),
         "vqe_12o_energy_grad": dict(parameters=len(sos), seconds=round(t_vqe, 3), energy=ev),
     }
     return out
+
+
+def cpu_sigma_shape(n, calls=1):
+    """The reference's OpenMP CPU sigma (oracle/_ref) at n orbitals, n/2 + n/2 electrons, timed
+    over ``calls`` calls after building the basis and network (no GPU involved)."""
+    import numpy as np
+    from oracle import glue
+    from oracle.ref import Reference, available
+    from binsim_amd.integrals import for_norb, one_body_layout, reference_layout
+    if not available():
+        raise SystemExit("oracle/_ref is not built")
+    R = Reference()
+    k = n // 2
+    t = time.perf_counter()
+    h, eri = for_norb(n)
+    terms = R.hamiltonian_terms(one_body_layout(h), reference_layout(eri), n)
+    flat = glue.flatten_network(glue.compress_by_svd(terms))
+    bp, dim = R.basis(n, k, k)
+    net = R.network(flat, n)
+    x = np.random.default_rng(1000 + n).uniform(-1, 1, dim)
+    setup = time.perf_counter() - t
+    log(f"cpu sigma n={n}: setup {setup:.1f} s, dim {dim}")
+    t = time.perf_counter()
+    for _ in range(calls):
+        R.hvec(bp, net, x)
+    dt = (time.perf_counter() - t) / calls
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(cpu_sigma=f"{n} orbitals {k}a{k}b", dim=dim, calls=calls, seconds_per_sigma=round(dt, 3),
+                algorithmic_GBps=round(sigma_bytes(n, k) / dt / 1e9, 3), threads=cores,
+                host_cpus=os.cpu_count(), setup_s=round(setup, 1),
+                kind="reference (oracle/_ref libotf_native, hvec_gather_contract_otf_f64, OpenMP)")
 
 
 def main():
@@ -265,8 +321,13 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--quick", action="store_true", help="skip the FCI solves and VQE checks")
-    ap.add_argument("--full", action="store_true",
-                    help="also run the 4-root FCI solves at 16 orbitals (plain and mixed; ~15 min)")
+    ap.add_argument("--no-fci16", action="store_true",
+                    help="skip the 16-orbital 4-root FCI comparison (plain vs mixed; minutes)")
+    ap.add_argument("--fci12", action="store_true",
+                    help="also run the 12-orbital 4-root comparison (plain vs mixed, tol 1e-8; ~20 s)")
+    ap.add_argument("--budget", type=float, default=500.0,
+                    help="seconds the default run may take; the FCI iteration caps and the VQE "
+                         "finite-difference sample are cut to fit")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pmc-unit", action="store_true",
                     help="one FP64 sigma and one plain copy of the same size (counter runs)")
@@ -275,6 +336,9 @@ def main():
     ap.add_argument("--davidson-only", type=int, default=None, metavar="MAXITER",
                     help="only the 16-orbital ground state to 1e-6: plain FP64, then mixed (block 1, "
                          "maxspace 32), each stopped after MAXITER iterations (600: to convergence)")
+    ap.add_argument("--cpu-sigma", type=int, default=None, metavar="N",
+                    help="only time the reference CPU sigma (oracle/_ref) at N orbitals, N/2+N/2 "
+                         "electrons, on the host cores (16 orbitals: about a minute on 16 cores)")
     ap.add_argument("--sigma-shape", default=None, metavar="N,K",
                     help="only time FP64 and single-vector FP32 sigma for N orbitals, K+K electrons")
     ap.add_argument("--tune", action="store_true",
@@ -286,6 +350,10 @@ def main():
               "would shard by destination alpha rows after an all-gather of x; not built)",
               file=sys.stderr)
         sys.exit(2)
+    t_begin = time.perf_counter()
+    if args.cpu_sigma:
+        print(json.dumps(cpu_sigma_shape(args.cpu_sigma)))
+        return
     import torch
     assert torch.cuda.is_available(), "no GPU visible"
     if args.vqe_only:
@@ -329,7 +397,8 @@ def main():
     gbs = bs / (ms * 1e-3) / 1e9
     log(f"sigma fp64: {ms:.2f} ms/step, {gbs:.1f} GB/s algorithmic")
     # block FP32 sigma (k = 4), per vector
-    xs = [seeded(torch, B.dim, 2000 + i) for i in range(4)]
+    g = torch.Generator(device="cuda").manual_seed(2000)
+    xs = [torch.rand(B.dim, dtype=torch.float64, device="cuda", generator=g) - 0.5 for _ in range(4)]
     ys = [torch.empty_like(v) for v in xs]
     F.hvec_block(xs, ys, 1)
     torch.cuda.synchronize()
@@ -340,6 +409,17 @@ def main():
     torch.cuda.synchronize()
     ms32 = e0.elapsed_time(e1) / 4
     del xs, ys
+    # the dominant kernel of one FP64 sigma is the alpha-beta source-lane kernel (~71% of the time,
+    # profiles/sigma_fp64_16o_kernel_trace.md); it is bound by LDS throughput, not HBM: per launch it
+    # reads one 8-byte integral per (determinant, alpha connection, beta slot) from LDS, writes
+    # the replicated integral row once per (alpha row, chunk of 512 source strings, connection),
+    # and adds each slot's sum into the LDS destination row once per chunk
+    from math import comb
+    Nn = comb(n, na)
+    sa, conn = na * (n - na), na * (n - na) + 1
+    lds_bytes = 8 * (B.dim * conn * sa + Nn * ((Nn + 511) // 512) * conn * 256 * 16 + B.dim * (sa + 1))
+    lds_peak = 128 * 256 * 2.4e9 / 1e9  # GB/s: 128 B/clk/CU x 256 CUs x 2.4 GHz
+    ab_ms = 148.2  # k_ab_src<double, 1, 8, 8, 16, ...> average, profiles/sigma_fp64_16o_kernel_trace.md
     result = {
         "metric": "FCI solve s + sigma GB/s, mixed vs FP64 at |dE|<1e-9 Ha; VQE adjoint grads/s",
         "value": round(gbs, 2),
@@ -352,20 +432,55 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4),
                      "traffic": SIGMA_TRAFFIC_16O,
+                     "achieved_is": "the algorithmic row-stream model B_sigma = 8 dim (2 + s_a + d_a) "
+                                    "divided by the time of one sigma, not measured HBM bytes",
                      "algorithmic_bytes_per_launch": bs,
+                     "traffic_source": "profiles/sigma_fp64_16o_pmc.md (FETCH_SIZE + WRITE_SIZE of "
+                                       "one sigma, each calibrated against a plain copy)",
+                     "measured_hbm_GBps": round(SIGMA_TRAFFIC_16O / (ms * 1e-3) / 1e9, 1),
+                     "measured_hbm_frac": round(SIGMA_TRAFFIC_16O / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
                      "kernel": "one sigma build (hvec_cuda): k_gauge_in, 2 row SpMMs (k_rbspmm, "
                                "source-range passes), k_ab_src, k_finalize on the default stream, "
-                               "timed with HIP events as one unit"},
+                               "timed with HIP events as one unit",
+                     "dominant_kernel": {
+                         "name": "k_ab_src<double, 1, 8, 8, 16, 1, 1, 256> (alpha-beta)",
+                         "avg_ms": ab_ms, "bound": "lds",
+                         "lds_bytes_per_launch": lds_bytes,
+                         "lds_GBps": round(lds_bytes / (ab_ms * 1e-3) / 1e9, 1),
+                         "lds_peak_GBps": lds_peak,
+                         "lds_frac": round(lds_bytes / (ab_ms * 1e-3) / 1e9 / lds_peak, 4),
+                         "source": "profiles/sigma_fp64_16o_kernel_trace.md, profiles/sigma_fp64_16o_pmc.md"}},
         "sigma_fp32_block4_ms_per_vector": round(ms32, 3),
         "tuning": tuning,
         "setup_s": round(setup_s, 3), "wall_s_timed": round(wall, 3),
     }
+
+    def left():
+        return args.budget - (time.perf_counter() - t_begin)
+
     if not args.quick:
-        if args.full:
-            result["fci_16o_4roots"] = fci_block(torch, B, H, 4, maxspace=64, tol=1e-6, maxiter=400)
-        del B, H, F, x, y
-        torch.cuda.empty_cache()
         from binsim_amd import davidson as D
+        # 16 orbitals, lowest 4 roots, residual 1e-6: plain FP64 vs mixed block (k = 4), both
+        # from the same P-space start.  Both need about 260 iterations here; the iteration cap
+        # only guards the time budget (measured per-iteration costs: plain ~4 FP64 sigmas +
+        # 0.26 s of subspace work, mixed ~4 FP32 block-sigma vectors + 0.1 s); a solve stopped
+        # by it reports converged = false.
+        if not args.no_fci16:
+            avail = max(0.0, left() - 60.0)  # VQE 16o, 12o checks and CPU baseline afterwards
+            it_plain, it_mixed = 4 * ms * 1e-3 + 0.26, 4 * ms32 * 1e-3 + 0.1
+            iters = min(600, int(avail / (it_plain + it_mixed)))
+            log(f"fci 16o: {avail:.0f} s available, iteration cap {iters}")
+            result["fci_16o_4roots"] = fci_block(torch, B, H, 4, maxspace=32, tol=1e-6,
+                                                 maxiter=max(iters, 1))
+        del x, y
+        del B, H, F
+        torch.cuda.empty_cache()
+        # VQE at 16 orbitals: 5,792 parameters, FD on 64 sampled ones (as many as the time allows)
+        fd_budget = max(5.0, left() - 40.0)
+        result["vqe_16o"] = vqe(torch, 16, 8, 8, nfd=64, seed_off=2000, amp=0.02, fd_budget_s=fd_budget)
+        torch.cuda.empty_cache()
+        # 12 orbitals (the CPU-runnable system): one root from HF (tol 1e-8) and the VQE
+        # energy + gradient with 1,818 parameters
         B0, H0, _, _ = build(12, 6, 6)
         e12 = AT.lookup(B0)
         for prec in (AT.FP64, AT.FP32):
@@ -375,18 +490,16 @@ def main():
         e, _, info = D.solve(B0, H0, nroots=1, mixed=False, tol=1e-8, guess=hf)
         result["fci_12o_1root_from_hf"] = dict(seconds=round(info["seconds"], 4), energy=float(e[0]),
                                                iterations=int(info["iterations"]))
-        result["fci_12o_4roots"] = fci_block(torch, B0, H0, 4, maxspace=32, tol=1e-8, maxiter=400)
-        del B0, H0
+        if args.fci12:
+            result["fci_12o_4roots"] = fci_block(torch, B0, H0, 4, maxspace=32, tol=1e-8, maxiter=400)
+        del B0, H0, hf
         result["vqe_12o"] = vqe(torch, 12, 6, 6, nfd=8)
         torch.cuda.empty_cache()
-        # the 16-orbital launch choices again (the 12-orbital ones differ)
-        AT.set_config(AT.FP64, tuning["fp64"])
-        AT.set_config(AT.FP32, tuning["fp32"])
-        result["vqe_16o"] = vqe(torch, 16, 8, 8, nfd=2, seed_off=2000, amp=0.02)
     if not args.no_cpu:
-        cb = cpu_baseline()
+        cb = cpu_baseline(budget_s=min(10.0, max(3.0, left() - 25.0)), extras=left() > 35.0)
         if cb is not None:
             result["cpu_baseline"] = cb
+    result["bench_seconds"] = round(time.perf_counter() - t_begin, 1)
     print(json.dumps(result))
 
 
