@@ -34,6 +34,7 @@ constexpr int AB_SRC_NJ = 512;   // source strings per chunk
 // integral entries one W fill covers (template NW): 256 for n*n <= 256, 512 for n*n <= 512
 constexpr int AB_SRC_XPF = 3;    // alpha connections of source elements in flight
 constexpr int AB_SRC_BINOM = 192; // binomial table entries: n <= 18 orbitals x (KM + 2) <= 10
+constexpr int AB_SRC_NTRI = 136;  // r <= t entries of a 16-orbital integral row (TRI layout)
 constexpr int AB_SRC_CONN = 88;   // alpha connections per row: k (n - k) <= 81 for n <= 18
 // slot rows of LDS gathers issued ahead of the FMAs (measured at 16 orbitals: 2 rows is no
 // faster than 1, 3 is 2-4% slower)
@@ -47,7 +48,12 @@ constexpr bool AB_SRC_HOIST = true;
 template <typename T, int NVS, int NW>
 using AbSrcRow = typename std::conditional<NW == 256 && NVS == 1, double, T>::type;
 
-template <typename T, int NVS, int KM, int VM, int R, int SPLIT, int DB, int NW>
+// TRI: the integral row is stored as its r <= t triangle, n(n+1)/2 entries ((pq|rt) = (pq|tr)),
+// which leaves room for R = 32 copies of 8-byte words beside the FP64 destination row: lanes l
+// and l + 16 then read different copies and the FP64 gathers are free of bank conflicts (with
+// 16 copies of the full row, lanes l and l + 16 collide whenever their entries share a parity).
+// The slot addresses are formed once per chunk and kept in VGPRs (single-vector FP64 form).
+template <typename T, int NVS, int KM, int VM, int R, int SPLIT, int DB, int NW, bool TRI = false>
 __global__ __launch_bounds__(AB_SRC_NJ *SPLIT) __attribute__((amdgpu_waves_per_eu(2 * SPLIT, 2 * SPLIT)))
 void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, const u32 *__restrict__ sxA,
               const u32 *__restrict__ sxB, const u64 *__restrict__ strsA, const u64 *__restrict__ strsB,
@@ -69,6 +75,8 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
     constexpr int LPE = 64 / EPW;                  // lanes per entry
     constexpr int CB = 16 / (int)sizeof(T);        // copies per 16-byte block
     constexpr int NBLK = R / CB;                   // blocks per entry
+    constexpr int NE = TRI ? AB_SRC_NTRI : NW;     // entries stored per replicated row
+    static_assert(!TRI || (DB == 1 && NVS == 1 && SPLIT == 1 && NW == 256), "triangular row: single-vector form");
     constexpr int BPL = NBLK / LPE;                // blocks stored per lane
     static_assert(KM % SPLIT == 0 && NWV * EPW == NW && LPE * EPW == 64 && NBLK % LPE == 0 &&
                   BPL >= 1, "fill layout");
@@ -77,8 +85,8 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
     const int n2 = n * n;
     // DB = 2: two replicated rows, the next connection's filled while the current one is read
     // (one barrier per connection instead of two)
-    T *Wr = reinterpret_cast<T *>(smem_raw);       // DB x NW * R replicated integral rows
-    T *Z = Wr + DB * NW * R;                       // 16 zeros (reads of unused slots)
+    T *Wr = reinterpret_cast<T *>(smem_raw);       // DB x NE * R replicated integral rows
+    T *Z = Wr + DB * NE * R;                       // 16 zeros (reads of unused slots)
     using ST = AbSrcRow<T, NVS, NW>;
     ST *S = reinterpret_cast<ST *>(Z + 16);        // nbp * NVS destination elements lo..
     constexpr int BW = KM + 2;
@@ -116,8 +124,21 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
         const int b = ((tid & 63) % LPE * BPL + q + (tid & 63) / LPE) % NBLK;
         foff[q] = (u32)((fe * R + b * CB) * sizeof(T));
     }
+    // TRI: fill entry fe < n(n+1)/2 is (a, b), a <= b, in row-major triangle order; it loads
+    // row element a*n + b, and threads with fe past the triangle store nothing
+    const int ntri = n * (n + 1) / 2;
+    int fsrc = fe;
+    if (TRI) {
+        int a = 0, start = 0;
+        while (a < n - 1 && start + (n - a) <= fe) {
+            start += n - a;
+            ++a;
+        }
+        fsrc = fe < ntri ? a * n + a + (fe - start) : 0;
+    }
     auto store_w = [&](T *W, T v) {
         const TV vv = v;
+        if (TRI && fe >= ntri) return;
 #pragma unroll
         for (int q = 0; q < BPL; ++q) *reinterpret_cast<TV *>(reinterpret_cast<unsigned char *>(W) + foff[q]) = vv;
     };
@@ -135,7 +156,7 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
         const u32 e = conn[single ? ca : 0];
         const T *er = single ? eri + (i64)(sx_r(e) * n + sx_s(e)) * n2 : dw + Ia * NW;
         neg = single && sx_neg(e);
-        pre = er[fe];
+        pre = er[fsrc];
     };
     // source elements of the round (chunk, connection ca); rounds are tracked by incremental
     // counters: a 64-bit division here would be expanded into vector code in the FMA phase.
@@ -181,47 +202,72 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
     bool live = false;
     // slots beyond (kB, vB) of the instantiated KM x VM grid read valid LDS words into
     // accumulators that are never scattered; their diagonal terms read the zeros Z
-    const u32 zero_off = (u32)(DB * NW * R * sizeof(T));
+    const u32 zero_off = (u32)(DB * NE * R * sizeof(T));
+    // TRI: byte offset of entry (a, b), a <= b, copy copy_off
+    auto tri_off = [&](u32 a, u32 b) {
+        return (u32)((a * (u32)n - (a * (a - 1u)) / 2u + (b - a)) * (u32)(R * sizeof(T))) + copy_off;
+    };
+    // TRI: the slot addresses, formed once per chunk, two 16-bit offsets per register (the
+    // triangle with its copies spans 34,816 bytes)
+    constexpr int VP = TRI ? (VM + 1) / 2 : 1;
+    u32 sa[TRI ? KH : 1][VP];
+    auto slot_addr = [&](int i, int iv) -> u32 {
+        const u32 w = sa[i % (TRI ? KH : 1)][(iv / 2) % VP];
+        return (iv & 1) ? w >> 16 : w & 0xffffu;
+    };
+    u32 op[KH];
+    // per chunk (at its first round): the thread's source string, its slot orbitals and
+    // addresses, zeroed sums
+    auto setup = [&](i64 c) {
+        Jb = c * AB_SRC_NJ + jl;
+        live = Jb < NB;
+        const u64 sl = strsB[live ? Jb : NB - 1];
+        const u64 s = live ? sl : 0ull;
+        u64 mo = s, mv = ~s & ((n < 64) ? ((1ull << n) - 1) : ~0ull);
+        for (int i = 0; i < half * KH; ++i) mo &= mo - 1;  // skip the other shares
+#pragma unroll
+        for (int i = 0; i < KH; ++i) {
+            const u32 p = mo ? (u32)__builtin_ctzll(mo) : 0u;
+            mo &= mo - 1;
+            op[i] = p;
+            if constexpr (!TRI) oo[i] = p * (u32)(n * R * sizeof(T));
+            dd[i] = half * KH + i < kB ? (TRI ? tri_off(p, p) : p * (u32)((n + 1) * R * sizeof(T)) + copy_off)
+                                       : zero_off;
+        }
+#pragma unroll
+        for (int iv = 0; iv < VM; ++iv) {
+            const u32 q = mv ? (u32)__builtin_ctzll(mv) : 0u;
+            mv &= mv - 1;
+            if constexpr (!TRI) vv[iv] = q * (u32)(R * sizeof(T)) + copy_off;
+            if constexpr (TRI) {
+#pragma unroll
+                for (int i = 0; i < KH; ++i) {
+                    const u32 a = op[i] < q ? tri_off(op[i], q) : tri_off(q, op[i]);
+                    sa[i][iv / 2] = (iv & 1) ? (sa[i][iv / 2] | (a << 16)) : a;
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < KH; ++i)
+#pragma unroll
+            for (int iv = 0; iv < VM; ++iv)
+#pragma unroll
+                for (int v = 0; v < NVS; ++v) acc[i][iv][v] = T(0);
+#pragma unroll
+        for (int v = 0; v < NVS; ++v) accd[v] = T(0);
+    };
     i64 chunk = 0;
     int ca = 0;
     for (i64 it = 0; it < nit; ++it, advance(chunk, ca)) {
-        if (ca == 0) {
-            Jb = chunk * AB_SRC_NJ + jl;
-            live = Jb < NB;
-            const u64 sl = strsB[live ? Jb : NB - 1];
-            const u64 s = live ? sl : 0ull;
-            u64 mo = s, mv = ~s & ((n < 64) ? ((1ull << n) - 1) : ~0ull);
-            for (int i = 0; i < half * KH; ++i) mo &= mo - 1;  // skip the other shares
-#pragma unroll
-            for (int i = 0; i < KH; ++i) {
-                const u32 p = mo ? (u32)__builtin_ctzll(mo) : 0u;
-                mo &= mo - 1;
-                oo[i] = p * (u32)(n * R * sizeof(T));
-                dd[i] = half * KH + i < kB ? p * (u32)((n + 1) * R * sizeof(T)) + copy_off : zero_off;
-            }
-#pragma unroll
-            for (int iv = 0; iv < VM; ++iv) {
-                const u32 q = mv ? (u32)__builtin_ctzll(mv) : 0u;
-                mv &= mv - 1;
-                vv[iv] = q * (u32)(R * sizeof(T)) + copy_off;
-            }
-#pragma unroll
-            for (int i = 0; i < KH; ++i)
-#pragma unroll
-                for (int iv = 0; iv < VM; ++iv)
-#pragma unroll
-                    for (int v = 0; v < NVS; ++v) acc[i][iv][v] = T(0);
-#pragma unroll
-            for (int v = 0; v < NVS; ++v) accd[v] = T(0);
-        }
+        if (ca == 0) setup(chunk);
         u32 wr_off = 0;  // byte offset of this round's replicated row
         if (DB == 2) {
             // this round's row (filled last round) is complete and the other buffer's last
             // readers are done: fill it with the next round's row, load the one after
             __syncthreads();
-            wr_off = (u32)((it & 1) * NW * R * sizeof(T));
+            wr_off = (u32)((it & 1) * NE * R * sizeof(T));
             if (it + 1 < nit) {
-                store_w(Wr + ((it + 1) & 1) * NW * R, pneg ? -pre : pre);
+                store_w(Wr + ((it + 1) & 1) * NE * R, pneg ? -pre : pre);
             }
             wca = wca + 1 == nca ? 0 : wca + 1;
             load_w(pre, pneg, wca);
@@ -248,7 +294,7 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
             const u32 lz = hoist ? 0u : *reinterpret_cast<const u32 *>(smem_raw + zero_off);
             u32 ob[KH];
 #pragma unroll
-            for (int i = 0; i < KH; ++i) ob[i] = oo[i] + wr_off + lz;
+            for (int i = 0; i < KH; ++i) ob[i] = TRI ? 0u : oo[i] + wr_off + lz;
             // software pipeline over the slot rows: row i+PD's VM gathers are issued before
             // row i's FMAs, so the LDS latency hides behind the FMAs of PD rows; the
             // scheduling barrier keeps the compiler from hoisting more rows of gathers (their
@@ -258,13 +304,15 @@ void k_ab_src(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
 #pragma unroll
             for (int d = 0; d < PD; ++d)
 #pragma unroll
-                for (int iv = 0; iv < VM; ++iv) wv[d][iv] = *reinterpret_cast<const T *>(smem_raw + ob[d] + vv[iv]);
+                for (int iv = 0; iv < VM; ++iv)
+                    wv[d][iv] = *reinterpret_cast<const T *>(smem_raw + (TRI ? slot_addr(d, iv) : ob[d] + vv[iv]));
 #pragma unroll
             for (int i = 0; i < KH; ++i) {
                 if (i + PD < KH) {
 #pragma unroll
                     for (int iv = 0; iv < VM; ++iv)
-                        wv[(i + PD) % (PD + 1)][iv] = *reinterpret_cast<const T *>(smem_raw + ob[i + PD] + vv[iv]);
+                        wv[(i + PD) % (PD + 1)][iv] = *reinterpret_cast<const T *>(
+                            smem_raw + (TRI ? slot_addr(i + PD, iv) : ob[i + PD] + vv[iv]));
                 }
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll

@@ -430,6 +430,7 @@ __global__ __launch_bounds__(1024) void k_ab_reg(const T *__restrict__ xt, T *__
 }
 
 #include "ab_src.hpp"
+#include "ab_cls.hpp"
 
 // ---------------------------------------------------------------------------------------
 // K5: sigma = E (s1 + s2T^T + E_core x~), FP64 output vectors
@@ -589,11 +590,12 @@ static void launch_ab_ept(DevHam *H, const T *xt, T *s1, int nvst, int v0, const
 template <typename T, int NVS>
 constexpr int ab_eptmax() { return (sizeof(T) == 8 && NVS >= 4) ? 4 : 8; }
 
-// alpha-beta kernel family (SigmaTuning::ab_kernel): 0 takes the first of source-lane,
-// register, chunked that applies; 1..3 start the search at that one
+// alpha-beta kernel family (SigmaTuning::ab_kernel): 0 takes the first of class-split (FP32
+// vector pairs, ab_cls.hpp), source-lane, register, chunked that applies; 1..3 start the search
+// at source-lane, register, chunked; 4 is 1 with FP32 vector pairs on the class-split kernel
 static int ab_kernel_pref(int prec) {
     const int p = tuning(prec).ab_kernel;
-    return p >= 0 && p <= 3 ? p : 0;
+    return p >= 0 && p <= 4 ? p : 0;
 }
 
 // bitwise reproducible sigma (the alpha-beta scatter runs wave by wave, ~15% slower); by
@@ -612,7 +614,7 @@ SigmaTuning &tuning(int precision) {
 }
 static bool deterministic(int prec) { return tuning(prec).deterministic != 0; }
 
-template <typename T, int NVS, int R, int NW>
+template <typename T, int NVS, int R, int NW, bool TRI = false>
 static bool try_ab_src_r(DevHam *H, const T *xt, T *s1, int nvst, int v0, const T *eri, hipStream_t st);
 
 // entries per integral row of the source-lane kernel for n orbitals (n*n <= 512)
@@ -659,10 +661,15 @@ static bool try_ab_src(DevHam *H, const T *xt, T *s1, int nvst, int v0, const T 
         return false;  // accumulators would not fit the register file
     } else {
         // R copies of the integral row: 32 lanes of 4-byte reads are served together, so 32
-        // copies make them conflict-free; 8-byte reads get 16 (LDS space).  (FP32 with 16
-        // copies at 16 orbitals: the same for two vectors, 9% slower for one; 64 copies 11%
-        // slower for one.  FP64 with 8 copies: sigma 208 -> 216 ms.)
-        if (H->n * H->n <= 256) return try_ab_src_r<T, NVS, sizeof(T) == 8 ? 16 : 32, 256>(H, xt, s1, nvst, v0, eri, st);
+        // copies make them conflict-free; 8-byte reads get 32 copies of the r <= t triangle in
+        // the single-buffer single-vector form (TRI), else 16 of the full row (LDS space).
+        // (FP32 with 16 copies at 16 orbitals: the same for two vectors, 9% slower for one; 64
+        // copies 11% slower for one.  FP64 with 8 copies: sigma 208 -> 216 ms.)
+        if (H->n * H->n <= 256) {
+            if constexpr (sizeof(T) == 8 && NVS == 1)
+                if (try_ab_src_r<T, NVS, 32, 256, true>(H, xt, s1, nvst, v0, eri, st)) return true;
+            return try_ab_src_r<T, NVS, sizeof(T) == 8 ? 16 : 32, 256>(H, xt, s1, nvst, v0, eri, st);
+        }
         // 17-22 orbitals: 512-entry rows, single-vector kernels only, 8 copies of the integral
         // row (FP32 at 18 orbitals / 7 electrons: 8 copies 4% faster than 16).  FP32: the 127 KB
         // destination row stays whole.  FP64: the 254 KB row is done in two halves, and at most
@@ -672,7 +679,7 @@ static bool try_ab_src(DevHam *H, const T *xt, T *s1, int nvst, int v0, const T 
     }
 }
 
-template <typename T, int NVS, int R, int NW>
+template <typename T, int NVS, int R, int NW, bool TRI>
 static bool try_ab_src_r(DevHam *H, const T *xt, T *s1, int nvst, int v0, const T *eri, hipStream_t st) {
     {
         DevBasis *B = H->basis;
@@ -690,12 +697,15 @@ static bool try_ab_src_r(DevHam *H, const T *xt, T *s1, int nvst, int v0, const 
         HIPOTF_REQUIRE(NW - n2 <= ERI_PAD, "internal: integral table padding too short for the row width");
         // + the binomial table C(p, j), p < n <= 18, j < KM + 2 <= 10; two replicated rows
         // (one barrier per connection) if tuned so and they fit
+        constexpr int NE = TRI ? AB_SRC_NTRI : NW;  // entries stored per integral row
         auto lds_for = [&](int db, i64 nbp) {
-            return ((size_t)db * NW * R + 16) * sizeof(T) + (size_t)nbp * NVS * sizeof(AbSrcRow<T, NVS, NW>) +
+            return ((size_t)db * NE * R + 16) * sizeof(T) + (size_t)nbp * NVS * sizeof(AbSrcRow<T, NVS, NW>) +
                    (AB_SRC_BINOM + AB_SRC_CONN) * 4;
         };
         const size_t lds_max = std::min<size_t>(lds_budget(), 160 * 1024);
         const int db = (tuning(prec).ab_db == 2 && lds_for(2, NB) <= lds_max) ? 2 : 1;
+        // the triangular row (TRI) serves the single-buffer form with one lane per source string
+        if (TRI && (db != 1 || split != 1 || n * (n + 1) / 2 > AB_SRC_NTRI)) return false;
         // destination-row parts: one, or (FP64 512-entry form) as few as fit the LDS
         int parts = 1;  // (the kernel checks scatter targets against its part in this form only)
         if (NW == 512 && sizeof(T) == 8)
@@ -737,19 +747,59 @@ static bool try_ab_src_r(DevHam *H, const T *xt, T *s1, int nvst, int v0, const 
         } else {
             auto pick = [&](auto split_c, auto db_c) {
                 constexpr int SPLIT = decltype(split_c)::value, DB = decltype(db_c)::value;
-                if (kB <= 4 && vB <= 4) go(k_ab_src<T, NVS, 4, 4, R, SPLIT, DB, NW>);
-                else if (kB <= 6 && vB <= 6) go(k_ab_src<T, NVS, 6, 6, R, SPLIT, DB, NW>);
+                if (kB <= 4 && vB <= 4) go(k_ab_src<T, NVS, 4, 4, R, SPLIT, DB, NW, TRI>);
+                else if (kB <= 6 && vB <= 6) go(k_ab_src<T, NVS, 6, 6, R, SPLIT, DB, NW, TRI>);
                 else if constexpr (sizeof(T) == 8 && NVS == 2) return false;  // 128 FP64 accumulators: spills
-                else if (kB <= 8 && vB <= 8) go(k_ab_src<T, NVS, 8, 8, R, SPLIT, DB, NW>);
+                else if (kB <= 8 && vB <= 8) go(k_ab_src<T, NVS, 8, 8, R, SPLIT, DB, NW, TRI>);
                 else return false;
                 return true;
             };
             using I1 = std::integral_constant<int, 1>;
             using I2 = std::integral_constant<int, 2>;
-            if (split == 2) return db == 2 ? pick(I2{}, I2{}) : pick(I2{}, I1{});
-            return db == 2 ? pick(I1{}, I2{}) : pick(I1{}, I1{});
+            if constexpr (TRI) {
+                return pick(I1{}, I1{});
+            } else {
+                if (split == 2) return db == 2 ? pick(I2{}, I2{}) : pick(I2{}, I1{});
+                return db == 2 ? pick(I1{}, I2{}) : pick(I1{}, I1{});
+            }
         }
     }
+}
+
+// the class-split FP32 kernel for vectors v0, v0 + 1 (ab_cls.hpp): 256-entry integral rows
+// (n <= 16), 1..8 beta electrons and holes, one class's FP64 destination elements of both
+// vectors in LDS; two launches (class T, class N)
+static bool try_ab_cls(DevHam *H, const float *xt, float *s1, int nvst, int v0, const float *eri,
+                       hipStream_t st) {
+    DevBasis *B = H->basis;
+    const DevSpin &A = *B->pA, &Bs = *B->pB;
+    const int n = H->n, kB = Bs.k, vB = Bs.n - Bs.k;
+    if (n * n > 256 || kB < 1 || vB < 1 || kB > 8 || vB > 8 || A.nsx > AB_SRC_CONN || n * 10 > AB_SRC_BINOM)
+        return false;
+    HIPOTF_REQUIRE(256 - n * n <= ERI_PAD, "internal: integral table padding too short for the row width");
+    constexpr int R = 32, NVS = 2;
+    const i64 NB = Bs.N, nN = NB * vB / n, nT = NB - nN;
+    const size_t smem = (256 * R + 16) * sizeof(float) + (size_t)std::max(nN, nT) * NVS * sizeof(double) +
+                        (AB_SRC_BINOM + AB_SRC_CONN) * 4;
+    if (smem > std::min<size_t>(lds_budget(), 160 * 1024)) return false;
+    const i64 nblocks = (A.N + 7) / 8 * 8;
+    HIPOTF_REQUIRE(nblocks < (1ll << 31), "alpha-beta grid too large");
+    const float *dw = diag_w_table<float>(H, st);
+    auto go = [&](auto kern, int ko, int vo, int kf) {
+        HIPOTF_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+        hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(AB_SRC_NJ), smem, st, xt, s1, nvst, v0,
+                           A.sx.as<u32>(), Bs.strs.as<u64>(), eri, dw, n, A.N, (int)NB, A.nsx, kB, ko, vo, kf);
+        HIPOTF_CHECK(hipGetLastError());
+    };
+    // class T: own slots (k-1) x (n-k), foreign k; class N: own k x (n-k-1), foreign n-k
+    if (kB == 8 && vB == 8) {
+        go(k_ab_cls<NVS, 7, 8, 8, R, true, true>, 7, 8, 8);
+        go(k_ab_cls<NVS, 8, 7, 8, R, false, true>, 8, 7, 8);
+    } else {
+        go(k_ab_cls<NVS, 8, 8, 8, R, true, false>, kB - 1, vB, kB);
+        go(k_ab_cls<NVS, 8, 8, 8, R, false, false>, kB, vB - 1, vB);
+    }
+    return true;
 }
 
 template <typename T, int NVS>
@@ -811,8 +861,15 @@ static void launch_ab(DevHam *H, const T *xt, T *s1, int nv, const T *eri, hipSt
     const size_t budget = std::min<size_t>(lds_budget(), 160 * 1024) - 4096;
     auto fits = [&](int s) { return (size_t)NB * s * sizeof(T) <= budget; };
     int v0 = 0;
-    if (ab_kernel_pref(sizeof(T) == 4) <= 1) {
+    const int pref = ab_kernel_pref(sizeof(T) == 4);
+    if (pref <= 1 || pref == 4) {
         while (v0 < nv) {
+            if constexpr (sizeof(T) == 4) {
+                if ((pref == 4 || pref == 0) && nv - v0 >= 2 && try_ab_cls(H, xt, s1, nv, v0, eri, st)) {
+                    v0 += 2;
+                    continue;
+                }
+            }
             const bool one = tuning(sizeof(T) == 4).ab_nvs == 1;
             if (!one && nv - v0 >= 2 && try_ab_src<T, 2>(H, xt, s1, nv, v0, eri, st)) { v0 += 2; continue; }
             if (try_ab_src<T, 1>(H, xt, s1, nv, v0, eri, st)) { v0 += 1; continue; }
@@ -872,9 +929,10 @@ static void sigma_group(DevHam *H, const SrcPack &src, const DstPack &dst, hipSt
 }
 
 template <typename T>
-static void sigma_t(DevHam *H, int nv, const double *const *src, double *const *dst, hipStream_t st) {
-    for (int g0 = 0; g0 < nv; g0 += MAXNV) {
-        const int m = std::min(MAXNV, nv - g0);
+static void sigma_t(DevHam *H, int nv, const double *const *src, double *const *dst, hipStream_t st,
+                    int maxnv) {
+    for (int g0 = 0; g0 < nv; g0 += maxnv) {
+        const int m = std::min(maxnv, nv - g0);
         SrcPack sp{};
         DstPack dp{};
         for (int i = 0; i < m; ++i) {
@@ -895,11 +953,13 @@ static void sigma_t(DevHam *H, int nv, const double *const *src, double *const *
     }
 }
 
-void sigma(DevHam *H, int nv, const double *const *src, double *const *dst, int prec, hipStream_t st) {
+void sigma(DevHam *H, int nv, const double *const *src, double *const *dst, int prec, hipStream_t st,
+           int maxnv) {
     HIPOTF_REQUIRE(H != nullptr, "Hamiltonian handle is NULL");
     HIPOTF_REQUIRE(nv >= 1, "need at least one vector");
-    if (prec == 0) sigma_t<double>(H, nv, src, dst, st);
-    else sigma_t<float>(H, nv, src, dst, st);
+    HIPOTF_REQUIRE(maxnv >= 1 && maxnv <= MAXNV, "vectors per sigma pass out of range");
+    if (prec == 0) sigma_t<double>(H, nv, src, dst, st, maxnv);
+    else sigma_t<float>(H, nv, src, dst, st, maxnv);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -57,11 +57,23 @@ def _properties(torch, n, k, det_check):
     F.hvec(y, hy)
     a, b = float(torch.dot(y, hx)), float(torch.dot(hy, x))
     assert abs(a - b) <= 1e-12 * max(abs(a), float(hx.norm() * y.norm()) * 1e-3)
-    # FP32 sigma of the same vector
+    # FP32 sigma of the same vector, alone and as the first of a pair (16 orbitals: the pair on
+    # the class-split kernel too)
     h32 = torch.empty_like(x)
     F.hvec_block([x], [h32], 1)
     assert float((h32 - hx).norm() / hx.norm()) < 2e-5
-    del y, hy, h32
+    p32, q32 = torch.empty_like(x), torch.empty_like(x)
+    kernels = (1, 4) if n <= 16 else (1,)
+    saved = AT.get_config(AT.FP32)
+    try:
+        for kern in kernels:
+            AT.set_config(AT.FP32, dict(saved, ab_kernel=kern, ab_nvs=2))
+            F.hvec_block([x, y], [p32, q32], 1)
+            assert float((p32 - hx).norm() / hx.norm()) < 2e-5
+            assert float((q32 - hy).norm() / hy.norm()) < 2e-5
+    finally:
+        AT.set_config(AT.FP32, saved)
+    del y, hy, h32, p32, q32
     # closed-shell HF determinant (index 0): diagonal and H e_0
     ehf = _hf_energy(h, eri, k)
     d = torch.zeros_like(x)

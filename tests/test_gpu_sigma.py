@@ -69,6 +69,28 @@ def test_block_sigma(torch_cuda, n, na, nb, k):
         assert rel_err(host(y), r) < 2e-5
 
 
+@pytest.mark.parametrize("n,na,nb", [(6, 3, 3), (8, 3, 5), (9, 4, 5), (10, 5, 2), (12, 6, 6), (12, 5, 6)])
+@pytest.mark.parametrize("k", [2, 3, 4])
+def test_class_split_fp32_pairs(torch_cuda, n, na, nb, k):
+    """FP32 block sigma with vector pairs on the class-split alpha-beta kernel (ab_kernel 4; an
+    odd last vector on the single-vector source-lane kernel) against the oracle."""
+    from binsim_amd import autotune as AT
+    S = oracle_system(n, na, nb)
+    xs = [seeded_vector(S.dim, 500 + j) for j in range(k)]
+    refs = [S.hvec(x) for x in xs]
+    B, H, F = _dev(n, na, nb)
+    xd = [gpu_vec(torch_cuda, x) for x in xs]
+    yd = [torch_cuda.empty_like(v) for v in xd]
+    saved = AT.get_config(AT.FP32)
+    try:
+        AT.set_config(AT.FP32, dict(saved, ab_kernel=4))
+        F.hvec_block(xd, yd, precision=1)
+    finally:
+        AT.set_config(AT.FP32, saved)
+    for y, r in zip(yd, refs):
+        assert rel_err(host(y), r) < 2e-5
+
+
 @pytest.mark.parametrize("n,na,nb", [(17, 1, 6), (18, 1, 7)])
 def test_sigma_512_entry_forms_both_precisions(torch_cuda, n, na, nb):
     """The source-lane kernel's 512-entry forms with at most 7 beta electrons and 11 holes:
