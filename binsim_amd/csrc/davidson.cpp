@@ -128,29 +128,38 @@ DavidsonOut davidson(DevHam *H, int nroots, int mode, int block, double tol, dou
     DavidsonOut out;
 
     // device memory of the whole solve, checked before anything is allocated: 2 * maxspace
-    // subspace vectors (V, AV), k residuals, the diagonal, and the sigma work buffers (4 per
-    // basis, dim * vectors per launch * word size; grown on first use).  The loop itself
+    // subspace vectors (V, AV), the diagonal, k residuals (plain mode; the block mode forms its
+    // residuals in free subspace slots), and the sigma work buffers (4 per basis, dim *
+    // vectors per pass * word size; grown on first use).  The block mode passes as many of its
+    // up to 8 vectors through one sigma as the memory left allows (nv_sigma).  The loop itself
     // allocates nothing: restarts and the FP32 -> FP64 switch combine the subspace in place.
+    int nv_sigma = 1;
     {
-        const int nv_sigma = mode == 0 ? 1 : std::min(k, 8);
         const size_t vec = (size_t)N * sizeof(double);
-        const size_t need_vec = vec * (2 * (size_t)maxspace + k + 1);
-        const size_t need_sig = 4 * vec * nv_sigma;  // FP64 phase: 4 buffers of dim * nv doubles
+        const size_t need_vec = vec * (2 * (size_t)maxspace + (mode == 0 ? k : 0) + 1);
         const size_t have_sig = std::min(std::min(H->basis->xt.bytes, H->basis->xtT.bytes),
                                          std::min(H->basis->s1.bytes, H->basis->s2T.bytes));
-        const size_t grow = need_sig / 4 > have_sig ? 4 * (need_sig / 4 - have_sig) : 0;
+        // FP64 phase: 4 buffers of dim * nv doubles (the FP32 phase needs half)
+        auto grow = [&](int nv) { return vec * nv > have_sig ? 4 * (vec * nv - have_sig) : (size_t)0; };
         size_t free_b = 0, total_b = 0;
         HIPOTF_CHECK(hipMemGetInfo(&free_b, &total_b));
-        if (need_vec + grow > free_b) {
+        if (need_vec + grow(1) > free_b) {
             char msg[320];
             std::snprintf(msg, sizeof msg,
                           "davidson: needs %.1f GB of device memory (%d + %d subspace vectors, %d residuals, "
                           "diagonal, %.1f GB more sigma work space) but %.1f GB are free; lower maxspace or block",
-                          (need_vec + grow) / 1e9, maxspace, maxspace, k, grow / 1e9, free_b / 1e9);
+                          (need_vec + grow(1)) / 1e9, maxspace, maxspace, mode == 0 ? k : 0, grow(1) / 1e9,
+                          free_b / 1e9);
             throw Error(msg);
         }
+        if (mode == 1)
+            for (int nv = std::min(k, 8); nv >= 1; --nv)
+                if (need_vec + grow(nv) <= free_b) {
+                    nv_sigma = nv;
+                    break;
+                }
     }
-    std::vector<DevBuf> Vb(maxspace), AVb(maxspace), R(k);
+    std::vector<DevBuf> Vb(maxspace), AVb(maxspace), R(mode == 0 ? k : 0);
     for (auto &b : Vb) b.alloc(N * sizeof(double));
     for (auto &b : AVb) b.alloc(N * sizeof(double));
     for (auto &b : R) b.alloc(N * sizeof(double));
@@ -249,7 +258,7 @@ DavidsonOut davidson(DevHam *H, int nroots, int mode, int block, double tol, dou
                 out.sigma64++;
             }
         } else {
-            sigma(H, count, (const double *const *)&V[first], &AV[first], phase, st);
+            sigma(H, count, (const double *const *)&V[first], &AV[first], phase, st, nv_sigma);
             if (phase == 1) out.sigma32 += count;
             else out.sigma64 += count;
         }
@@ -299,12 +308,34 @@ DavidsonOut davidson(DevHam *H, int nroots, int mode, int block, double tol, dou
                             (const double *const *)V.data(), vecs + (size_t)i * N, st);
         }
     };
-    for (int it = 1; it <= maxiter; ++it) {
-        out.iterations = it;
+    auto rayleigh_ritz = [&]() {
         std::vector<double> a((size_t)m * m);
         for (int r = 0; r < m; ++r)
             for (int c = 0; c < m; ++c) a[(size_t)r * m + c] = heff[(size_t)r * maxspace + c];
         jacobi_eigh(m, a, w, C);
+    };
+    // restart to the nkeep lowest Ritz vectors of the last Rayleigh-Ritz step (jl/davidson.jl:82-114):
+    // V_i, AV_i <- the Ritz combinations, in place (no second copy of the kept vectors)
+    auto restart = [&]() {
+        const int keep = std::min(nkeep, m);
+        std::vector<double> Cm((size_t)m * keep);
+        for (int j = 0; j < m; ++j)
+            for (int i = 0; i < keep; ++i) Cm[(size_t)j * keep + i] = C[(size_t)j * m + i];
+        rotate_inplace(N, m, keep, Cm.data(), V.data(), st);
+        rotate_inplace(N, m, keep, Cm.data(), AV.data(), st);
+        m = keep;
+        ritz_in_V = true;
+        std::fill(heff.begin(), heff.end(), 0.0);
+        set_cols(0, m);
+    };
+    for (int it = 1; it <= maxiter; ++it) {
+        out.iterations = it;
+        rayleigh_ritz();
+        if (mode == 1 && m + std::min(k, m) > maxspace) {
+            // the block mode forms its residuals in the free slots V[m..m+kk): restart first
+            restart();
+            rayleigh_ritz();
+        }
         const int kk = std::min(k, m);  // Ritz pairs with residuals this iteration
         std::vector<std::vector<double>> coef(kk, std::vector<double>(m));
         double rmax = 0.0;  // over the wanted roots
@@ -313,12 +344,12 @@ DavidsonOut davidson(DevHam *H, int nroots, int mode, int block, double tol, dou
         coef_last = coef;
         ritz_in_V = false;
         if (mode == 1) {
-            // all kk residuals in one pass over the subspace
+            // all kk residuals in one pass over the subspace, into the free slots V[m..m+kk)
             std::vector<double> Cm((size_t)m * kk);
             std::vector<double *> Rp(kk);
             for (int j = 0; j < m; ++j)
                 for (int i = 0; i < kk; ++i) Cm[(size_t)j * kk + i] = coef[i][j];
-            for (int i = 0; i < kk; ++i) Rp[i] = R[i].as<double>();
+            for (int i = 0; i < kk; ++i) Rp[i] = V[m + i];
             block_combine(N, m, kk, Cm.data(), w.data(), (const double *const *)V.data(),
                           (const double *const *)AV.data(), Rp.data(), rn.data(), st);
         } else {
@@ -381,33 +412,19 @@ DavidsonOut davidson(DevHam *H, int nroots, int mode, int block, double tol, dou
         std::vector<int> todo;
         for (int i = 0; i < kk; ++i)
             if (rn[i] >= cur_tol) todo.push_back(i);
-        if (m + (int)todo.size() > maxspace) {
-            // restart to the nkeep lowest Ritz vectors (jl/davidson.jl:82-114)
-            const int keep = std::min(nkeep, m);
-            std::vector<std::vector<double>> ck(keep, std::vector<double>(m));
-            for (int i = 0; i < keep; ++i)
-                for (int j = 0; j < m; ++j) ck[i][j] = C[(size_t)j * m + i];
-            // V_i, AV_i <- the Ritz combinations, in place (no second copy of the kept vectors)
-            std::vector<double> Cm((size_t)m * keep);
-            for (int j = 0; j < m; ++j)
-                for (int i = 0; i < keep; ++i) Cm[(size_t)j * keep + i] = ck[i][j];
-            rotate_inplace(N, m, keep, Cm.data(), V.data(), st);
-            rotate_inplace(N, m, keep, Cm.data(), AV.data(), st);
-            m = keep;
-            ritz_in_V = true;
-            std::fill(heff.begin(), heff.end(), 0.0);
-            set_cols(0, m);
-        }
+        if (mode == 0 && m + (int)todo.size() > maxspace) restart();  // (the block mode restarted above)
         const int first = m;
         if (mode == 1) {
-            // preconditioned residuals of up to 8 pairs at once, orthogonalised against the
-            // subspace by two passes of block classical Gram-Schmidt, then against each other
+            // preconditioned residuals of up to 8 pairs at once (residual i sits in V[m + i];
+            // the slot pointers are permuted so the b-th pair of todo moves to V[m + b]),
+            // orthogonalised against the subspace by two passes of block classical Gram-Schmidt,
+            // then against each other
             int b = 0;
             std::vector<double> nb0;
             for (int i : todo) {
                 if (m + b >= maxspace || b >= 8) break;
+                std::swap(V[m + b], V[m + i]);  // i >= b: earlier swaps only touched lower slots
                 double *t = V[m + b];
-                HIPOTF_CHECK(hipMemcpyAsync(t, R[i].p, N * sizeof(double), hipMemcpyDeviceToDevice, st));
                 precondition(N, t, diag.as<double>(), w[i], std::max(rn[i] * 1e-2, 1e-4), st);
                 nb0.push_back(std::sqrt(dot(N, t, t, st)));
                 ++b;

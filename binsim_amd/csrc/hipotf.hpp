@@ -232,9 +232,10 @@ struct SigmaTuning {
 SigmaTuning &tuning(int precision);  // 0: FP64 kernels, 1: FP32 kernels
 constexpr int TUNING_FIELDS = 8;
 
-// sigma: dst_v = H src_v for nv vectors; prec 0 = float64 work, 1 = float32 work
+// sigma: dst_v = H src_v for nv vectors; prec 0 = float64 work, 1 = float32 work; at most
+// maxnv (1..8) vectors share one pass (its work buffers hold dim * maxnv words each)
 void sigma(DevHam *H, int nv, const double *const *src, double *const *dst, int prec,
-           hipStream_t st);
+           hipStream_t st, int maxnv = 8);
 void diag_elements(DevHam *H, double *diags, hipStream_t st);  // accumulates
 // one FP64 sigma with the four work buffers carved from the library's static device scratch
 // array (no device allocation; every table must already exist); at most
