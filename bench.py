@@ -8,7 +8,7 @@ seeded vector; ``value`` is its algorithmic HBM rate  B_sigma / t  with
 B_sigma = 8 * dim * (2 + s_a + d_a) bytes (s_a = 64 singles, d_a = 784 doubles per string).
 
 Reported beside it (one run each, outside the timed steps, in this order; --budget seconds
-bound the whole run, 500 by default):
+bound the whole run, 520 by default; about 450 s are used):
   * FCI, lowest 4 roots at the 16-orbital workload to residual 1e-6: plain FP64 Davidson (one
     FP64 sigma per correction vector) and mixed-precision block Davidson (k = 4: FP32 block
     sigma, FP64 Rayleigh-Ritz, FP64 refinement), wall seconds, sigma counts and per-root |dE|;
@@ -44,6 +44,9 @@ PEAK_HBM_GBS = 8000.0
 # dispatches of one sigma), each scaled by what the same counter reads for a plain copy kernel
 # of 1.325 GB in the same run (profiles/sigma_fp64_16o_pmc.md)
 SIGMA_TRAFFIC_16O = 266.4e9
+# average launch time of the FP64 alpha-beta kernel in one 16-orbital sigma, from the kernel
+# trace of `bench.py --steps 3 --warmup 1 --quick --no-cpu` (profiles/sigma_fp64_16o_kernel_trace.md)
+AB_SRC_MS_16O = 144.7
 
 
 def log(*a):
@@ -118,8 +121,9 @@ def fci_block(torch, B, H, nroots, maxspace, tol, maxiter, maxiter_mixed=None):
 
 
 def sigma_shape(torch, n, k, steps, warmup):
-    """FP64 and single-vector FP32 sigma for n orbitals, k+k electrons, on a seeded vector,
-    wall clock around ``steps`` calls after ``warmup`` (stored launch choices where present)."""
+    """FP64, single-vector FP32 and 4-vector FP32 sigma for n orbitals, k+k electrons, on seeded
+    vectors, wall clock around ``steps`` calls after ``warmup`` (stored launch choices where
+    present); the 4-vector figure's algorithmic GB/s counts all four vectors."""
     from binsim_amd import autotune as AT
     B, H, _, F = build(n, k, k)
     entry = AT.lookup(B)
@@ -127,8 +131,12 @@ def sigma_shape(torch, n, k, steps, warmup):
         AT.apply(entry)
     x = seeded(torch, B.dim, 1000 + n)
     y, y32 = torch.empty_like(x), torch.empty_like(x)
+    g = torch.Generator(device="cuda").manual_seed(3000 + n)
+    xs = [x] + [torch.rand(B.dim, dtype=torch.float64, device="cuda", generator=g) - 0.5 for _ in range(3)]
+    ys = [torch.empty_like(x) for _ in range(4)]
     out = {"sigma_shape": f"{n} orbitals {k}a{k}b", "dim": B.dim, "steps": steps}
-    for name, fn, w in (("fp64", lambda: F.hvec(x, y), 8), ("fp32", lambda: F.hvec_block([x], [y32], 1), 4)):
+    for name, fn, w in (("fp64", lambda: F.hvec(x, y), 8), ("fp32", lambda: F.hvec_block([x], [y32], 1), 4),
+                        ("fp32_block4", lambda: F.hvec_block(xs, ys, 1), 16)):
         for _ in range(warmup):
             fn()
         torch.cuda.synchronize()
@@ -139,7 +147,9 @@ def sigma_shape(torch, n, k, steps, warmup):
         dt = (time.perf_counter() - t) / steps
         out[name] = dict(seconds=round(dt, 4), algorithmic_GBps=round(sigma_bytes(n, k) * w / 8 / dt / 1e9, 1))
         log(f"sigma {name} n={n} k={k}: {dt * 1e3:.1f} ms")
+    out["fp32_block4"]["seconds_per_vector"] = round(out["fp32_block4"]["seconds"] / 4, 4)
     out["fp32_rel_dev"] = float((y32 - y).norm() / y.norm())
+    out["fp32_block4_rel_dev"] = float((ys[0] - y).norm() / y.norm())
     return out
 
 
@@ -167,10 +177,11 @@ def fci16_ground(torch, maxiter):
 
 
 def vqe(torch, n, na, nb, nfd, seed_off=2000, amp=0.05, stored_tuning=False, fd_budget_s=None):
-    """Fused energy + adjoint gradient, then central differences (h = 1e-6) of the energy-only
-    entry point on nfd parameters sampled with seed + 3000 (binsim_amd.vqe.fd_gradient: a running
-    prefix state, so each evaluation sweeps only the generators from the sampled one on).  With
-    fd_budget_s the sample is cut short when the time runs out (fd_params says how many ran)."""
+    """Fused energy + adjoint gradient, then central differences (h = 1e-6) on nfd parameters
+    sampled with seed + 3000 (binsim_amd.vqe.fd_gradient: a running prefix state, so each trial
+    state sweeps only the generators from the sampled one on, and one sigma per parameter for
+    the energy difference).  With fd_budget_s the sample is cut short when the time runs out
+    (fd_params says how many ran)."""
     import numpy as np
     from binsim_amd.vqe import fd_gradient
     B, H, P, F = build(n, na, nb, pool=True)
@@ -204,16 +215,10 @@ def vqe(torch, n, na, nb, nfd, seed_off=2000, amp=0.05, stored_tuning=False, fd_
     back_to_hf = float((psi - hf).abs().max())
     rng = np.random.default_rng(SEED_BASE + 3000)
     sample = np.sort(rng.choice(npar, min(nfd, npar), replace=False))
-    work = [torch.empty_like(psi) for _ in range(3)]
-    done, fd = [], []
+    work = [torch.empty_like(psi) for _ in range(4)]
     t_fd = time.perf_counter()
-    # in chunks of 8 (sorted positions), so the budget check runs between chunks
-    for c0 in range(0, len(sample), 8):
-        if fd_budget_s is not None and time.perf_counter() - t_fd > fd_budget_s:
-            break
-        chunk = sample[c0:c0 + 8]
-        fd.extend(fd_gradient(F, idx, th, chunk, 1e-6, hf, work).tolist())
-        done.extend(chunk.tolist())
+    fd = fd_gradient(F, idx, th, sample, 1e-6, hf, work, energy=e, budget_s=fd_budget_s).tolist()
+    done = sample[:len(fd)].tolist()
     t_fd = time.perf_counter() - t_fd
     dev = float(np.max(np.abs(np.array(fd) - g[done]))) if done else None
     log(f"vqe n={n}: {dt * 1e3:.1f} ms per energy+grad, fd dev {dev} over {len(done)} ({t_fd:.1f} s)")
@@ -221,6 +226,45 @@ def vqe(torch, n, na, nb, nfd, seed_off=2000, amp=0.05, stored_tuning=False, fd_
     return dict(parameters=npar, seconds_per_eval=round(dt, 5), grads_per_s=round(1.0 / dt, 3),
                 energy=e, dH2=d2, fd_max_dev=dev, fd_params=len(done), fd_seconds=round(t_fd, 2),
                 fd_h=1e-6, fd_sample_seed=SEED_BASE + 3000, state_back_to_hf_max_dev=back_to_hf)
+
+
+def batchgrad_time(torch, n, reps=3):
+    """batchgrad_cuda (the ADAPT pool gradient: <l|G_g e^{theta_g G_g}|r> for every generator)
+    at n orbitals, n/2 + n/2 electrons: the fused gradient-only pass against one launch per
+    generator (sweep_mode 2), seconds per call (HIP-synchronised wall clock), and the largest
+    difference between the two."""
+    from binsim_amd import autotune as AT
+    B, H, P, F = build(n, n // 2, n // 2, pool=True)
+    entry = AT.lookup(B)
+    if entry is not None:
+        AT.apply(entry)
+    from binsim_amd.ansatz import ucc_spin_orbitals
+    ng = len(ucc_spin_orbitals(n, n // 2, n // 2))
+    g = torch.Generator(device="cuda").manual_seed(77 + n)
+    l = torch.rand(B.dim, dtype=torch.float64, device="cuda", generator=g) - 0.5
+    r = torch.rand(B.dim, dtype=torch.float64, device="cuda", generator=g) - 0.5
+    l /= l.norm()
+    r /= r.norm()
+    th = (torch.rand(ng, dtype=torch.float64, device="cuda", generator=g) - 0.5) * 0.04
+    out = {"batchgrad": f"{n} orbitals {n // 2}a{n // 2}b", "generators": ng}
+    res = {}
+    base = AT.get_config(AT.FP64)
+    for name, mode in (("fused", base.get("sweep_mode", 0)), ("per_generator", 2)):
+        AT.set_config(AT.FP64, dict(base, sweep_mode=mode))
+        gd = torch.zeros(ng, dtype=torch.float64, device="cuda")
+        F.batchgrad(l, r, gd, th)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            F.batchgrad(l, r, gd, th)
+        torch.cuda.synchronize()
+        out[name + "_seconds"] = round((time.perf_counter() - t) / reps, 4)
+        res[name] = gd.clone()
+        log(f"batchgrad n={n} {name}: {out[name + '_seconds']} s")
+    AT.set_config(AT.FP64, base)
+    out["max_abs_diff"] = float((res["fused"] - res["per_generator"]).abs().max())
+    out["speedup"] = round(out["per_generator_seconds"] / out["fused_seconds"], 2)
+    return out
 
 
 def cpu_baseline(budget_s=10.0, extras=True, dav_iters=200):
@@ -325,7 +369,7 @@ def main():
                     help="skip the 16-orbital 4-root FCI comparison (plain vs mixed; minutes)")
     ap.add_argument("--fci12", action="store_true",
                     help="also run the 12-orbital 4-root comparison (plain vs mixed, tol 1e-8; ~20 s)")
-    ap.add_argument("--budget", type=float, default=500.0,
+    ap.add_argument("--budget", type=float, default=520.0,
                     help="seconds the default run may take; the FCI iteration caps and the VQE "
                          "finite-difference sample are cut to fit")
     ap.add_argument("--no-cpu", action="store_true")
@@ -336,6 +380,12 @@ def main():
     ap.add_argument("--davidson-only", type=int, default=None, metavar="MAXITER",
                     help="only the 16-orbital ground state to 1e-6: plain FP64, then mixed (block 1, "
                          "maxspace 32), each stopped after MAXITER iterations (600: to convergence)")
+    ap.add_argument("--batchgrad", type=int, choices=(12, 16), default=None, metavar="N",
+                    help="only time the pool gradient (batchgrad_cuda) at N orbitals: fused pass vs "
+                         "one launch per generator")
+    ap.add_argument("--cpu-only", action="store_true",
+                    help="only the CPU baseline at 12 orbitals (reference sigma, Davidson from HF to "
+                         "1e-8, VQE energy + gradient) on the host cores")
     ap.add_argument("--cpu-sigma", type=int, default=None, metavar="N",
                     help="only time the reference CPU sigma (oracle/_ref) at N orbitals, N/2+N/2 "
                          "electrons, on the host cores (16 orbitals: about a minute on 16 cores)")
@@ -354,6 +404,9 @@ def main():
     if args.cpu_sigma:
         print(json.dumps(cpu_sigma_shape(args.cpu_sigma)))
         return
+    if args.cpu_only:
+        print(json.dumps(cpu_baseline(budget_s=10.0, extras=True)))
+        return
     import torch
     assert torch.cuda.is_available(), "no GPU visible"
     if args.vqe_only:
@@ -361,6 +414,9 @@ def main():
         r = vqe(torch, m, m // 2, m // 2, nfd=0, seed_off=2000, amp=0.05 if m == 12 else 0.02,
                 stored_tuning=True)
         print(json.dumps({"vqe": m, **r}))
+        return
+    if args.batchgrad:
+        print(json.dumps(batchgrad_time(torch, args.batchgrad)))
         return
     if args.davidson_only:
         print(json.dumps(fci16_ground(torch, args.davidson_only)))
@@ -409,17 +465,20 @@ def main():
     torch.cuda.synchronize()
     ms32 = e0.elapsed_time(e1) / 4
     del xs, ys
-    # the dominant kernel of one FP64 sigma is the alpha-beta source-lane kernel (~71% of the time,
-    # profiles/sigma_fp64_16o_kernel_trace.md); it is bound by LDS throughput, not HBM: per launch it
-    # reads one 8-byte integral per (determinant, alpha connection, beta slot) from LDS, writes
-    # the replicated integral row once per (alpha row, chunk of 512 source strings, connection),
-    # and adds each slot's sum into the LDS destination row once per chunk
+    # the dominant kernel of one FP64 sigma is the alpha-beta source-lane kernel (~70% of the
+    # time, profiles/sigma_fp64_16o_kernel_trace.md); it is bound by LDS traffic and issue, not
+    # HBM.  Its LDS bytes per launch: one 8-byte integral gather per (determinant, alpha
+    # connection, beta slot) plus k diagonal gathers, the replicated r <= t integral row (32
+    # copies) written once per (alpha row, chunk of 512 source strings, connection), and a
+    # read + write per FP64 scatter add (one per slot and the diagonal, per chunk)
     from math import comb
     Nn = comb(n, na)
     sa, conn = na * (n - na), na * (n - na) + 1
-    lds_bytes = 8 * (B.dim * conn * sa + Nn * ((Nn + 511) // 512) * conn * 256 * 16 + B.dim * (sa + 1))
+    ntri = n * (n + 1) // 2
+    lds_bytes = 8 * (B.dim * conn * (sa + nb) + Nn * ((Nn + 511) // 512) * conn * ntri * 32
+                     + 2 * B.dim * (sa + 1))
     lds_peak = 128 * 256 * 2.4e9 / 1e9  # GB/s: 128 B/clk/CU x 256 CUs x 2.4 GHz
-    ab_ms = 148.2  # k_ab_src<double, 1, 8, 8, 16, ...> average, profiles/sigma_fp64_16o_kernel_trace.md
+    ab_ms = AB_SRC_MS_16O
     result = {
         "metric": "FCI solve s + sigma GB/s, mixed vs FP64 at |dE|<1e-9 Ha; VQE adjoint grads/s",
         "value": round(gbs, 2),
@@ -443,7 +502,7 @@ def main():
                                "source-range passes), k_ab_src, k_finalize on the default stream, "
                                "timed with HIP events as one unit",
                      "dominant_kernel": {
-                         "name": "k_ab_src<double, 1, 8, 8, 16, 1, 1, 256> (alpha-beta)",
+                         "name": "k_ab_src<double, 1, 8, 8, 32, 1, 1, 256, true> (alpha-beta)",
                          "avg_ms": ab_ms, "bound": "lds",
                          "lds_bytes_per_launch": lds_bytes,
                          "lds_GBps": round(lds_bytes / (ab_ms * 1e-3) / 1e9, 1),
@@ -461,13 +520,13 @@ def main():
     if not args.quick:
         from binsim_amd import davidson as D
         # 16 orbitals, lowest 4 roots, residual 1e-6: plain FP64 vs mixed block (k = 4), both
-        # from the same P-space start.  Both need about 260 iterations here; the iteration cap
-        # only guards the time budget (measured per-iteration costs: plain ~4 FP64 sigmas +
-        # 0.26 s of subspace work, mixed ~4 FP32 block-sigma vectors + 0.1 s); a solve stopped
-        # by it reports converged = false.
+        # from the same P-space start.  Both need about 256 iterations here; the iteration cap
+        # only guards the time budget (measured per-iteration costs, averaged over the solves:
+        # plain ~4 FP64 sigmas + 0.17 s of subspace work, mixed ~4 FP32 block-sigma vectors +
+        # 0.06 s); a solve stopped by it reports converged = false.
         if not args.no_fci16:
-            avail = max(0.0, left() - 60.0)  # VQE 16o, 12o checks and CPU baseline afterwards
-            it_plain, it_mixed = 4 * ms * 1e-3 + 0.26, 4 * ms32 * 1e-3 + 0.1
+            avail = max(0.0, left() - 65.0)  # VQE 16o, 12o checks and CPU baseline afterwards
+            it_plain, it_mixed = 4 * ms * 1e-3 + 0.17, 4 * ms32 * 1e-3 + 0.06
             iters = min(600, int(avail / (it_plain + it_mixed)))
             log(f"fci 16o: {avail:.0f} s available, iteration cap {iters}")
             result["fci_16o_4roots"] = fci_block(torch, B, H, 4, maxspace=32, tol=1e-6,

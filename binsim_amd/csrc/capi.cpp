@@ -609,7 +609,7 @@ int hipotf_set_tuning(int precision, const int32_t *v, int n) {
     API_TRY
     HIPOTF_REQUIRE(precision == 0 || precision == 1, "tuning: precision must be 0 or 1");
     HIPOTF_REQUIRE(v != nullptr && n == TUNING_FIELDS, "tuning: expected 8 values");
-    HIPOTF_REQUIRE(v[0] >= 0 && v[0] <= 3, "tuning: ab_kernel must be 0..3");
+    HIPOTF_REQUIRE(v[0] >= 0 && v[0] <= 4, "tuning: ab_kernel must be 0..4");
     HIPOTF_REQUIRE(v[1] == 1 || v[1] == 2, "tuning: ab_split must be 1 or 2");
     HIPOTF_REQUIRE(v[2] == 1 || v[2] == 2, "tuning: ab_db must be 1 or 2");
     HIPOTF_REQUIRE(v[3] == 1 || v[3] == 2, "tuning: ab_nvs must be 1 or 2");
@@ -736,17 +736,16 @@ double backgrad_cuda_2d(void *basis, void *net, int64_t idx, double theta, doubl
     return backgrad_cuda(basis, net, idx, theta, lp, rp);
 }
 
-// grads[g] = grad(g, thetas[g], lp, rp) for every generator; thetas / grads are device arrays
+// grads[g] = grad(g, thetas[g], lp, rp) for every generator (src/include/cuda/batchgrad.cuh:106-378);
+// thetas / grads are device arrays.  One gradient-only pass over the sweeps' fused segments:
+// the generators sharing an alpha pattern (or a row kind) read their rows once per segment
 void batchgrad_cuda(void *basis, void *net, const double *thetas, const double *lp, const double *rp,
                     double *grads) {
     API_TRY
     DevBasis *B = as_basis(basis);
     DevPool *P = as_net(net)->pool_for(B);
-    HIPOTF_REQUIRE(thetas && grads, "NULL argument");
-    const i64 ng = (i64)P->gens.size();
-    std::vector<double> th(ng);
-    if (ng) HIPOTF_CHECK(hipMemcpy(th.data(), thetas, ng * sizeof(double), hipMemcpyDeviceToHost));
-    for (i64 g = 0; g < ng; ++g) rot_grad(P, g, th[g], lp, rp, grads + g, S0());
+    HIPOTF_REQUIRE(thetas && grads && lp && rp, "NULL argument");
+    fused_grads(P, thetas, lp, rp, grads, S0());
     API_CATCH_VOID
 }
 
@@ -791,12 +790,17 @@ int hipotf_vqe_energy(void *basis, void *ham, void *pool, int64_t nparams, const
                       const double *thetas, double *psi, double *lam, double *out3) {
     API_TRY
     DevBasis *B = as_basis(basis);
-    DevHam *H = as_net(ham)->ham_for(B);
     DevPool *P = as_net(pool)->pool_for(B);
-    HIPOTF_REQUIRE(psi && lam && out3 && (nparams == 0 || (idxs && thetas)), "NULL argument");
+    HIPOTF_REQUIRE(psi && lam && (nparams == 0 || (idxs && thetas)), "NULL argument");
     const i64 dim = B->host.dim;
     hipStream_t st = S0();
     fused_sweeps(P, nparams, idxs, thetas, psi, lam, true, false, nullptr, st);
+    if (!ham) {  // forward sweep only
+        HIPOTF_CHECK(hipStreamSynchronize(st));
+        return 0;
+    }
+    HIPOTF_REQUIRE(out3, "NULL argument");
+    DevHam *H = as_net(ham)->ham_for(B);
     const double *src = psi;
     double *dst = lam;
     sigma(H, 1, &src, &dst, 0, st);

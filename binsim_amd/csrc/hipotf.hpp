@@ -279,6 +279,12 @@ struct DevPool {
 void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, double *psi, double *lam,
                   bool forward, bool backward, double *dres, hipStream_t st);
 
+// gradient-only pass over every pool generator (jl/vqe.jl's batch gradient for ADAPT):
+// grads[g] = <lp| G_g e^{theta_g G_g} |rp>, the generators grouped into the sweeps' fused
+// segments; thetas and grads are device arrays
+void fused_grads(DevPool *P, const double *thetas, const double *lp, const double *rp, double *grads,
+                 hipStream_t st);
+
 DevPool *make_dev_pool(DevBasis *b, i64 nops, const int32_t *offsets, const int32_t *orbs);
 
 void rot_expm(DevPool *P, i64 idx, double theta, double *vec, hipStream_t st);

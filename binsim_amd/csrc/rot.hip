@@ -16,6 +16,7 @@
 //   expm     : x_u <- c x_u - t s x_w ,  x_w <- t s x_u + c x_w
 //   grad     : g += l_u(-s r_u + t c r_w) + l_w(-s r_w - t c r_u)
 //   backgrad : l <- R(-theta) l ; g += [same form with new l, old r] ; r <- R(-theta) r
+#include <numeric>
 #include "hipotf.hpp"
 
 #include <algorithm>
@@ -312,6 +313,10 @@ void rot_grad(DevPool *P, i64 idx, double theta, const double *lp, const double 
     HIPOTF_CHECK(hipGetLastError());
 }
 
+// rot_grad with the angle on the device
+static void rot_grad_dev(DevPool *P, i64 idx, const double *theta, const double *lp, const double *rp,
+                         double *dres, hipStream_t st);
+
 void rot_backgrad(DevPool *P, i64 idx, double theta, double *lp, double *rp, double *dres,
                   hipStream_t st) {
     PairView v;
@@ -377,7 +382,7 @@ constexpr int seg_mode(int k, int NST) { return k < NST ? 0 : (k == NST ? 1 : 2)
 // memory and are updated in place, one generator after the other, with the launch sized to
 // one block per CU so that the rows in flight stay in the L2 / Infinity Cache across the
 // segment's generators.  All rows staged (NST = 1 forward, 2 adjoint) is the LDS form.
-template <bool BACK, int NST>
+template <bool BACK, int NST, bool GRAD = false>
 __global__ __launch_bounds__(1024) void k_seg_row(double *__restrict__ l, double *__restrict__ r,
                                                              i64 nrows, i64 nel, const u64 *__restrict__ row_strs,
                                                              const u64 *__restrict__ el_strs, int row_is_alpha,
@@ -415,7 +420,12 @@ __global__ __launch_bounds__(1024) void k_seg_row(double *__restrict__ l, double
             const u64 es = el_strs[u];
             const bool neg = row_is_alpha ? jw_sign(sg.g, rs, es) : jw_sign(sg.g, es, rs);
             const double ts = neg ? -sg.s : sg.s;
-            if (!BACK) {
+            if (GRAD) {
+                // <l| G e^{theta G} |r> of this generator alone (k_grad), nothing stored
+                const double tc = neg ? -sg.c : sg.c;
+                const double lu = xl.ld(u), lw = xl.ld(w), ru = xr.ld(u), rw = xr.ld(w);
+                acc += lu * (-sg.s * ru + tc * rw) + lw * (-sg.s * rw - tc * ru);
+            } else if (!BACK) {
                 const double xu = xl.ld(u), xw = xl.ld(w);
                 xl.st(u, sg.c * xu - ts * xw);
                 xl.st(w, ts * xu + sg.c * xw);
@@ -436,14 +446,14 @@ __global__ __launch_bounds__(1024) void k_seg_row(double *__restrict__ l, double
         }
         __syncthreads();  // (also orders this generator's global updates before the next one's reads)
     }
-    stage(false);
+    if (!GRAD) stage(false);
 }
 
 // one block = one alpha pair (u_a, w_a) of the segment's alpha pattern; rows of NB elements
 // (the u and w strings of one pattern are disjoint, so blocks own disjoint row pairs).  NST of
 // the rows (l_u, l_w, then r_u, r_w) are staged in LDS, the others updated in place in global
 // memory, as in k_seg_row; NST = 2 forward / 4 adjoint is the LDS form
-template <bool BACK, int NST>
+template <bool BACK, int NST, bool GRAD = false>
 __global__ __launch_bounds__(1024) void k_seg_pair(double *__restrict__ l, double *__restrict__ r,
                                                               const int32_t *__restrict__ alist, i64 npairs, i64 NB,
                                                               const u64 *__restrict__ strsA,
@@ -483,7 +493,11 @@ __global__ __launch_bounds__(1024) void k_seg_pair(double *__restrict__ l, doubl
             const int32_t ub = sg.lst[2 * j], wb = sg.lst[2 * j + 1];
             const bool neg = jw_sign(sg.g, as, strsB[ub]);
             const double ts = neg ? -sg.s : sg.s;
-            if (!BACK) {
+            if (GRAD) {
+                const double tc = neg ? -sg.c : sg.c;
+                const double lu = l0.ld(ub), lw = l1.ld(wb), ru = r0.ld(ub), rw = r1.ld(wb);
+                acc += lu * (-sg.s * ru + tc * rw) + lw * (-sg.s * rw - tc * ru);
+            } else if (!BACK) {
                 const double xu = l0.ld(ub), xw = l1.ld(wb);
                 l0.st(ub, sg.c * xu - ts * xw);
                 l1.st(wb, ts * xu + sg.c * xw);
@@ -504,7 +518,7 @@ __global__ __launch_bounds__(1024) void k_seg_pair(double *__restrict__ l, doubl
         }
         __syncthreads();  // (also orders this generator's global updates before the next one's reads)
     }
-    stage(false);
+    if (!GRAD) stage(false);
 }
 
 // dres[k] = sum_b partial[k * nb + b], one block per generator of the segment
@@ -604,10 +618,56 @@ static std::vector<Segment> plan_sweep(DevPool *P, const i64 *idxs, i64 n, int n
     return segs;
 }
 
-// Forward expm sweep then adjoint sweep, both fused where possible.  psi / lam device vectors;
-// dres[i] receives the raw backgrad value of sweep position i.
-void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, double *psi, double *lam,
-                  bool forward, bool backward, double *dres, hipStream_t st) {
+// c, s of sweep position i from the device angles (the gradient-only pass: no angles on the host)
+__global__ __launch_bounds__(256) void k_seg_cs(SegGen *__restrict__ d, const double *__restrict__ th, i64 n) {
+    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        d[i].c = cos(th[i]);
+        d[i].s = sin(th[i]);
+    }
+}
+
+// k_grad with the angle read on the device
+__global__ __launch_bounds__(256) void k_grad_dev(PairView v, GenDev g, const double *__restrict__ th,
+                                                  const double *__restrict__ l, const double *__restrict__ r,
+                                                  double *__restrict__ partial) {
+    const double c = cos(*th), s = sin(*th);
+    const i64 tot = v.na * v.nb;
+    double acc = 0.0;
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (i64)gridDim.x * blockDim.x) {
+        i64 u, w;
+        u64 ua, ub;
+        pair_at(v, i, u, w, ua, ub);
+        const double tc = jw_sign(g, ua, ub) ? -c : c;
+        const double ru = r[u], rw = r[w];
+        acc += l[u] * (-s * ru + tc * rw) + l[w] * (-s * rw - tc * ru);
+    }
+    const double t = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+static void rot_grad_dev(DevPool *P, i64 idx, const double *theta, const double *lp, const double *rp,
+                         double *dres, hipStream_t st) {
+    PairView v;
+    GenDev g;
+    views(P, idx, v, g);
+    const i64 tot = v.na * v.nb;
+    const int nb = tot ? grid_for(tot) : 1;
+    if (tot) {
+        hipLaunchKernelGGL(k_grad_dev, dim3(nb), dim3(256), 0, st, v, g, theta, lp, rp, P->partial.as<double>());
+        HIPOTF_CHECK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, P->partial.as<double>(), tot ? nb : 0, 1.0, dres);
+    HIPOTF_CHECK(hipGetLastError());
+}
+
+// Forward expm sweep then adjoint sweep, both fused where possible, or (grad) the gradient-only
+// pass: dres[i] = <psi| G_i e^{theta_i G_i} |lam> for every position, each generator on the same
+// two vectors, with the angles on the device (thetas_dev).  psi / lam device vectors; dres[i]
+// receives the raw backgrad (grad) value of sweep position i.
+static void sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, const double *thetas_dev,
+                   double *psi, double *lam, bool forward, bool backward, bool grad, double *dres,
+                   hipStream_t st) {
     DevBasis *B = P->basis;
     const i64 NA = B->pA->N, NB = B->pB->N, dim = NA * NB;
     for (i64 i = 0; i < n; ++i)
@@ -628,14 +688,20 @@ void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, doub
                 const PairList &L = (s.kind == SEG_ROW_B) ? P->lists[G.pa] : P->lists[G.pb];
                 d.lst = L.uw.as<int32_t>();
                 d.len = L.len;
-                d.c = std::cos(thetas[i]);
-                d.s = std::sin(thetas[i]);
+                d.c = thetas ? std::cos(thetas[i]) : 0.0;
+                d.s = thetas ? std::sin(thetas[i]) : 0.0;
             }
     };
+    bool transposed = false;  // grad: the beta-major copies are made once (the vectors do not change)
     auto run = [&](const std::vector<Segment> &segs, bool back) {
         fill(segs);
         P->segdesc.ensure((size_t)std::max<i64>(n, 1) * sizeof(SegGen));
         if (n) HIPOTF_CHECK(hipMemcpyAsync(P->segdesc.p, host.data(), n * sizeof(SegGen), hipMemcpyHostToDevice, st));
+        if (grad && n) {
+            hipLaunchKernelGGL(k_seg_cs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                               P->segdesc.as<SegGen>(), thetas_dev, n);
+            HIPOTF_CHECK(hipGetLastError());
+        }
         const SegGen *dg = P->segdesc.as<SegGen>();
         const int nvec = back ? 2 : 1;
         // segments with rows in global memory: one block per CU (LDS reserved for that where no
@@ -648,7 +714,8 @@ void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, doub
                 for (int kk = 0; kk < s.count; ++kk) {
                     const int k = back ? s.count - 1 - kk : kk;
                     const i64 i = s.first + k;
-                    if (back) rot_backgrad(P, idxs[i], thetas[i], psi, lam, dres + i, st);
+                    if (grad) rot_grad_dev(P, idxs[i], thetas_dev + i, psi, lam, dres + i, st);
+                    else if (back) rot_backgrad(P, idxs[i], thetas[i], psi, lam, dres + i, st);
                     else rot_expm(P, idxs[i], thetas[i], psi, st);
                 }
                 continue;
@@ -661,8 +728,11 @@ void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, doub
                     P->tl.ensure(dim * sizeof(double));
                     P->tr.ensure(dim * sizeof(double));
                     dim3 g((unsigned)((NB + 31) / 32), (unsigned)((NA + 31) / 32));
-                    hipLaunchKernelGGL(k_transpose, g, dim3(256), 0, st, psi, P->tl.as<double>(), NA, NB);
-                    if (back) hipLaunchKernelGGL(k_transpose, g, dim3(256), 0, st, lam, P->tr.as<double>(), NA, NB);
+                    if (!transposed) {
+                        hipLaunchKernelGGL(k_transpose, g, dim3(256), 0, st, psi, P->tl.as<double>(), NA, NB);
+                        if (back) hipLaunchKernelGGL(k_transpose, g, dim3(256), 0, st, lam, P->tr.as<double>(), NA, NB);
+                        transposed = grad;
+                    }
                     l = P->tl.as<double>();
                     r = P->tr.as<double>();
                 }
@@ -684,7 +754,11 @@ void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, doub
                     hipLaunchKernelGGL(kern, dim3((unsigned)nrows), dim3(bd), smem, st, l, rr, nrows, nel, rstr,
                                        estr, A ? 1 : 0, dg + s.first, s.count, pp, s.sp);
                 };
-                if (back) {
+                if (grad) {
+                    if (s.nst == 0) go_row(k_seg_row<true, 0, true>, r, part);
+                    else if (s.nst == 1) go_row(k_seg_row<true, 1, true>, r, part);
+                    else go_row(k_seg_row<true, 2, true>, r, part);
+                } else if (back) {
                     if (s.nst == 0) go_row(k_seg_row<true, 0>, r, part);
                     else if (s.nst == 1) go_row(k_seg_row<true, 1>, r, part);
                     else go_row(k_seg_row<true, 2>, r, part);
@@ -693,7 +767,7 @@ void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, doub
                     else go_row(k_seg_row<false, 1>, (double *)nullptr, (double *)nullptr);
                 }
                 HIPOTF_CHECK(hipGetLastError());
-                if (!A) {
+                if (!A && !grad) {
                     dim3 g((unsigned)((NA + 31) / 32), (unsigned)((NB + 31) / 32));
                     hipLaunchKernelGGL(k_transpose, g, dim3(256), 0, st, P->tl.as<double>(), psi, NB, NA);
                     if (back) hipLaunchKernelGGL(k_transpose, g, dim3(256), 0, st, P->tr.as<double>(), lam, NB, NA);
@@ -723,7 +797,13 @@ void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, doub
                                        np, NB, B->pA->strs.as<u64>(), B->pB->strs.as<u64>(), dg + s.first,
                                        s.count, pp, s.sp);
                 };
-                if (back) {
+                if (grad) {
+                    if (s.nst == 0) go_pair(k_seg_pair<true, 0, true>, lam, part);
+                    else if (s.nst == 1) go_pair(k_seg_pair<true, 1, true>, lam, part);
+                    else if (s.nst == 2) go_pair(k_seg_pair<true, 2, true>, lam, part);
+                    else if (s.nst == 3) go_pair(k_seg_pair<true, 3, true>, lam, part);
+                    else go_pair(k_seg_pair<true, 4, true>, lam, part);
+                } else if (back) {
                     if (s.nst == 0) go_pair(k_seg_pair<true, 0>, lam, part);
                     else if (s.nst == 1) go_pair(k_seg_pair<true, 1>, lam, part);
                     else if (s.nst == 2) go_pair(k_seg_pair<true, 2>, lam, part);
@@ -741,8 +821,27 @@ void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, doub
             }
         }
     };
+    if (grad) {
+        run(plan_sweep(P, idxs, n, 2), true);
+        return;
+    }
     if (forward) run(plan_sweep(P, idxs, n, 1), false);
     if (backward) run(plan_sweep(P, idxs, n, 2), true);
+}
+
+void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, double *psi, double *lam,
+                  bool forward, bool backward, double *dres, hipStream_t st) {
+    sweeps(P, n, idxs, thetas, nullptr, psi, lam, forward, backward, false, dres, st);
+}
+
+void fused_grads(DevPool *P, const double *thetas, const double *lp, const double *rp, double *grads,
+                 hipStream_t st) {
+    const i64 n = (i64)P->gens.size();
+    std::vector<i64> idxs(n);
+    std::iota(idxs.begin(), idxs.end(), 0);
+    // the vectors are only read in the gradient-only pass
+    sweeps(P, n, idxs.data(), nullptr, thetas, const_cast<double *>(lp), const_cast<double *>(rp), false,
+           false, true, grads, st);
 }
 
 // ---------------------------------------------------------------------------------------

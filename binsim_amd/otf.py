@@ -204,6 +204,16 @@ class HipOTF_Functions:
         L.check(rc == 0, "hipotf_vqe_energy")
         return float(out3[0]), float(out3[1]), float(out3[2])
 
+    def vqe_sweep(self, idxs, thetas, psi, lam):
+        """Forward sweep only (hipotf_vqe_energy with no Hamiltonian): psi <- U(theta) psi,
+        the fused rotation segments of the VQE evaluation; lam is workspace."""
+        idx = _np(idxs, np.int64)
+        th = _np(thetas, np.float64)
+        rc = L.load().hipotf_vqe_energy(
+            self.basis.ptr, None, self.pool.ptr, len(th), idx.ctypes.data_as(L._i64p),
+            th.ctypes.data_as(L._dp), _ptr(psi), _ptr(lam), None)
+        L.check(rc == 0, "hipotf_vqe_energy")
+
 
 def sync_device():
     L.call_void("sync_device_cuda")

@@ -136,7 +136,8 @@ int hipotf_vqe_energy_grad(void *basis, void *ham, void *pool, int64_t nparams, 
                            double *out3);
 /* energy only (the evaluations of jl/vqe.jl:1-21 without the gradient: line searches,
  * finite-difference checks): forward sweep, lam = H psi, out3 as above; psi: in = reference
- * state, out = U(theta)|ref> (not uncomputed). */
+ * state, out = U(theta)|ref> (not uncomputed).  With ham == NULL only the forward sweep runs
+ * (lam is workspace, out3 untouched and may be NULL). */
 int hipotf_vqe_energy(void *basis, void *ham, void *pool, int64_t nparams, const int64_t *idxs,
                       const double *thetas, double *psi, double *lam, double *out3);
 

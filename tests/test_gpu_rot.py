@@ -104,14 +104,27 @@ def test_vqe_config2_matches_reference_fixture(torch_cuda, sweep_mode):
     assert abs(float(host(psi)[0]) - 1.0) < 1e-12 and abs(nrm - 1.0) < 1e-12
 
 
-def test_batchgrad(torch_cuda):
-    n, na, nb = 6, 3, 3
+@pytest.mark.parametrize("n,na,nb,mode", [(6, 3, 3, 0), (8, 3, 5, 0), (8, 3, 5, 2), (12, 6, 6, 0),
+                                         (12, 6, 6, 3)])
+def test_batchgrad(torch_cuda, n, na, nb, mode):
+    """Pool gradient of every generator at one (l, r) pair (batchgrad_cuda, the fused
+    gradient-only pass; sweep_mode 2 runs one launch per generator, 3 stages one row and half of
+    the next) against the oracle's grad of each generator."""
+    from binsim_amd import autotune as AT
+    print(f"batchgrad n={n} na={na} nb={nb} sweep_mode={mode}", flush=True)
     S = oracle_system(n, na, nb, pool=True)
     B, H, P, F, sos = _dev(n, na, nb)
-    rng = np.random.default_rng(3)
+    rng = np.random.default_rng(3 + n)
     l, r = rng.uniform(-1, 1, S.dim), rng.uniform(-1, 1, S.dim)
+    l /= np.linalg.norm(l)
+    r /= np.linalg.norm(r)
     th = rng.uniform(-0.5, 0.5, len(sos))
     gd = torch_cuda.zeros(len(sos), dtype=torch_cuda.float64, device="cuda")
-    F.batchgrad(gpu_vec(torch_cuda, l), gpu_vec(torch_cuda, r), gd, gpu_vec(torch_cuda, th))
+    saved = AT.get_config(AT.FP64)
+    try:
+        AT.set_config(AT.FP64, dict(saved, sweep_mode=mode))
+        F.batchgrad(gpu_vec(torch_cuda, l), gpu_vec(torch_cuda, r), gd, gpu_vec(torch_cuda, th))
+    finally:
+        AT.set_config(AT.FP64, saved)
     ref = np.array([S.grad(i, th[i], l, r) for i in range(len(sos))])
     assert np.max(np.abs(host(gd) - ref)) < 1e-12

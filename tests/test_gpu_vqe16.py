@@ -48,7 +48,7 @@ def test_vqe_16_orbitals_adjoint_gradient_matches_fd(torch_cuda):
     rng = np.random.default_rng(SEED_BASE + 3000)
     sample = sorted(int(rng.choice(v)) for v in kinds.values())
     assert len(sample) == 5
-    work = [torch.empty_like(psi) for _ in range(3)]
+    work = [torch.empty_like(psi) for _ in range(4)]
     fd = fd_gradient(F, idx, th, sample, 1e-6, hf, work)
     dev = np.abs(fd - g[sample])
     print(f"vqe 16 orbitals: fd max dev {dev.max():.3e} at {sample}", flush=True)
@@ -57,3 +57,7 @@ def test_vqe_16_orbitals_adjoint_gradient_matches_fd(torch_cuda):
     psi.copy_(hf)
     e1, d21, nrm1 = F.vqe_energy(idx, th, psi, lam)
     assert abs(e1 - e) < 1e-12 and abs(d21 - d2) < 1e-9 and abs(nrm1 - 1.0) < 1e-12
+    # the sweep-only form (no Hamiltonian) leaves the same state U(theta)|HF>
+    psi2 = hf.clone()
+    F.vqe_sweep(idx, th, psi2, lam)
+    assert float((psi2 - psi).abs().max()) < 1e-14
