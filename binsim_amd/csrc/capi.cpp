@@ -73,14 +73,18 @@ struct HostNet {
 // basis-dependent tables are built on first use with a basis and cached under its uid
 // ------------------------------------------------------------------------------------------
 struct DevNet {
-    int kind = 0;  // 1 Hamiltonian, 2 pool
+    int kind = 0;  // 1 Hamiltonian (integrals), 2 pool, 3 generic Pauli-group network
     int n = 0;
+    DevGenNet *gen = nullptr;  // kind 3
+    ~DevNet() { destroy_dev_gennet(gen); }
     double ecore = 0.0;
     std::vector<double> h, eri;
     std::vector<int32_t> offsets, orbs;
     std::map<i64, std::unique_ptr<DevHam>> hams;
     std::map<i64, std::unique_ptr<DevPool>> pools;
     DevHam *ham_for(DevBasis *b) {
+        HIPOTF_REQUIRE(kind != 3, "this network is evaluated group by group (it is not a spin-free two-body "
+                                  "Hamiltonian): only hvec_cuda and get_diags_elements_cuda take it");
         HIPOTF_REQUIRE(kind == 1, "operator handle is not a Hamiltonian");
         auto it = hams.find(b->uid);
         if (it != hams.end()) return it->second.get();
@@ -90,6 +94,8 @@ struct DevNet {
     }
     // the Hamiltonian's tables for b if they exist (no allocation), else an error
     DevHam *ham_prepared(DevBasis *b) {
+        HIPOTF_REQUIRE(kind != 3, "this network is evaluated group by group (it is not a spin-free two-body "
+                                  "Hamiltonian): only hvec_cuda and get_diags_elements_cuda take it");
         HIPOTF_REQUIRE(kind == 1, "operator handle is not a Hamiltonian");
         auto it = hams.find(b->uid);
         HIPOTF_REQUIRE(it != hams.end(), "the Hamiltonian's device tables for this basis are not built: call "
@@ -554,8 +560,30 @@ void *build_networkdev_f64(void *host_net) {
     auto D = std::make_unique<DevNet>();
     bool has_diag = N.by_x.count(0) > 0;
     if (has_diag) {
-        decode_hamiltonian(N, *D);
-        verify_hamiltonian(N, *D);
+        // a spin-free two-body Hamiltonian decodes into integrals (the fast path); any other
+        // network with a diagonal group is kept as its Pauli groups and evaluated as such
+        try {
+            decode_hamiltonian(N, *D);
+            verify_hamiltonian(N, *D);
+        } catch (const Error &) {
+            D = std::make_unique<DevNet>();
+            D->kind = 3;
+            D->n = N.norb;
+            std::vector<u32> ax, bx;
+            std::vector<int> rank;
+            std::vector<std::vector<u32>> zas, zbs;
+            std::vector<std::vector<double>> was, wbs;
+            for (const HostGroup &G : N.groups) {
+                ax.push_back(G.ax);
+                bx.push_back(G.bx);
+                rank.push_back(G.rank);
+                zas.push_back(G.zas);
+                zbs.push_back(G.zbs);
+                was.push_back(G.wa);
+                wbs.push_back(G.wb);
+            }
+            D->gen = make_dev_gennet(N.norb, ax, bx, rank, zas, zbs, was, wbs);
+        }
     } else {
         decode_pool(N, *D);
         verify_pool(N, *D);
@@ -565,6 +593,13 @@ void *build_networkdev_f64(void *host_net) {
 }
 
 void destroy_networkdev_f64(void *net) { delete static_cast<DevNet *>(net); }
+
+int hipotf_network_kind(void *net) {
+    API_TRY
+    HIPOTF_REQUIRE(net, "operator handle is NULL");
+    return as_net(net)->kind;
+    API_CATCH(0)
+}
 
 void *hipotf_build_ham_f64(int64_t norb, const double *one_body_mo, const double *two_body_mo,
                            double energy_nuc) {
@@ -596,7 +631,7 @@ int hipotf_prepare(void *basis, void *net) {
     DevBasis *B = as_basis(basis);
     DevNet *N = as_net(net);
     if (N->kind == 1) N->ham_for(B);
-    else N->pool_for(B);
+    else if (N->kind == 2) N->pool_for(B);
     HIPOTF_CHECK(hipDeviceSynchronize());
     return 0;
     API_CATCH(1)
@@ -658,6 +693,10 @@ int64_t hipotf_num_generators(void *net) {
 void hvec_cuda(void *basis, void *net, const double *src, double *dst) {
     API_TRY
     DevBasis *B = as_basis(basis);
+    if (as_net(net)->kind == 3) {
+        gennet_hvec(as_net(net)->gen, B, src, dst, S0());
+        return;
+    }
     DevHam *H = as_net(net)->ham_for(B);
     sigma(H, 1, &src, &dst, 0, S0());
     API_CATCH_VOID
@@ -691,6 +730,10 @@ int hipotf_hvec_block(void *basis, void *net, int nvec, const double *const *src
 void get_diags_elements_cuda(void *basis, void *net, double *diags) {
     API_TRY
     DevBasis *B = as_basis(basis);
+    if (as_net(net)->kind == 3) {
+        gennet_diag(as_net(net)->gen, B, diags, S0());
+        return;
+    }
     diag_elements(as_net(net)->ham_for(B), diags, S0());
     API_CATCH_VOID
 }

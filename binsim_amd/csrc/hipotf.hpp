@@ -287,6 +287,17 @@ void fused_grads(DevPool *P, const double *thetas, const double *lp, const doubl
 
 DevPool *make_dev_pool(DevBasis *b, i64 nops, const int32_t *offsets, const int32_t *orbs);
 
+// generic Pauli-group network (network.hip): any operator given as the reference's SVD network,
+// evaluated group by group; group g: flips (ax, bx), rank, z masks and weights w[r * nz + k]
+struct DevGenNet;
+DevGenNet *make_dev_gennet(int n, const std::vector<u32> &ax, const std::vector<u32> &bx,
+                           const std::vector<int> &rank, const std::vector<std::vector<u32>> &zas,
+                           const std::vector<std::vector<u32>> &zbs, const std::vector<std::vector<double>> &was,
+                           const std::vector<std::vector<double>> &wbs);
+void destroy_dev_gennet(DevGenNet *D);
+void gennet_hvec(DevGenNet *D, DevBasis *B, const double *x, double *y, hipStream_t st);  // y = H x
+void gennet_diag(DevGenNet *D, DevBasis *B, double *diag, hipStream_t st);               // accumulates
+
 void rot_expm(DevPool *P, i64 idx, double theta, double *vec, hipStream_t st);
 // grad: returns device pointer to the scalar result (written asynchronously)
 void rot_grad(DevPool *P, i64 idx, double theta, const double *lp, const double *rp, double *dres,

@@ -56,11 +56,18 @@ void *build_basisdev_f64(void *host_basis);               /* cuotf.cu:25-30 */
 void destroy_basisdev_f64(void *basis);                   /* cuotf.cu:14-23 */
 /* build_networkdev_f64 (cuotf.cu:43-48): decodes a host network from build_network_otf_f64.
  * A network with a diagonal group is read as a two-body Hamiltonian (integrals recovered by
- * probing matrix elements); otherwise every group must be +-(T - T^dagger) of one excitation.
- * The decode is re-checked against the network's elements on probe determinants; a network it
- * does not represent exactly (spin models, S^2 penalties, non-two-body terms) returns NULL. */
+ * probing matrix elements), the fast path; otherwise every group must be +-(T - T^dagger) of
+ * one excitation (a generator pool).  The decode is re-checked against the network's elements
+ * on probe determinants.  A network with a diagonal group that it does not represent exactly
+ * (spin models, S^2 penalties, non-two-body terms) is kept as its Pauli groups and evaluated
+ * group by group (hvec.cuh:6-446's algorithm; hvec_cuda and get_diags_elements_cuda only); a
+ * pool network that fails returns NULL. */
 void *build_networkdev_f64(void *host_net);
 void destroy_networkdev_f64(void *net);                   /* cuotf.cu:32-41 */
+/* which form a device operator handle took (no reference counterpart; for callers that want to
+ * know whether the fast path applies): 1 two-body Hamiltonian in integral form, 2 generator
+ * pool, 3 generic Pauli-group network; 0 on error */
+int hipotf_network_kind(void *net);
 /* hvec_cuda (cuotf.cu:50-60): dst = H src (dst overwritten, hvec.cuh:439) */
 void hvec_cuda(void *basis, void *net, const double *src, double *dst);
 /* get_diags_elements_cuda (cuotf.cu:62-71): diags[i] += H_ii (accumulates) */
@@ -94,7 +101,8 @@ void *hipotf_build_ham_f64(int64_t norb, const double *one_body_mo, const double
 void *hipotf_build_pool_f64(int64_t norb, int64_t nops, const int32_t *offsets,
                             const int32_t *orbs);
 int64_t hipotf_num_generators(void *net);
-/* builds the basis-dependent device tables of an operator now instead of on first use */
+/* builds the basis-dependent device tables of an operator now instead of on first use (the
+ * integral-form Hamiltonian and pools; a generic network builds its tables in hvec_cuda) */
 int hipotf_prepare(void *basis, void *net);
 /* launch choices of the sigma kernels, process-wide (no reference counterpart; the
  * reference's tiles are compile-time, src/include/ham/otf.hpp:6-9 and cuda/common.cuh:7-12):

@@ -338,6 +338,87 @@ def fermion_operator(term):
     return multiply([ladder(pos, bool(cre)) for pos, cre in term])
 
 
+def add_terms(*ops, coeffs=None, tol: float = EPS2):
+    """linearcombine of binqubitaabb.jl: sum_i c_i op_i, merged by (x, z)."""
+    coeffs = coeffs or [1.0] * len(ops)
+    return _reduce([(x, z, w * c) for op, w in zip(ops, coeffs) for x, z, c in op], tol)
+
+
+def qubit_operator(terms, coeff: float):
+    """binqubitaabb.jl:436-463 QubitOperatorAABB: the Pauli string of (site, "X"/"Y"/"Z") on the
+    interleaved spin-orbital qubits, Y = (x and z set), coefficient i^popc(x & z) * coeff (real
+    strings only: an odd number of Y is rejected)."""
+    x = z = 0
+    for pos, pauli in terms:
+        site = 1 << int(pos)
+        if pauli == "X":
+            x |= site
+        elif pauli == "Y":
+            x |= site
+            z |= site
+        elif pauli == "Z":
+            z |= site
+        else:
+            raise ValueError(f"invalid Pauli name {pauli}")
+    k = popc(x & z)
+    if k & 1:
+        raise ValueError("imaginary coefficient: odd number of Y")
+    return [(x, z, (-1.0 if (k // 2) & 1 else 1.0) * float(coeff))]
+
+
+def spin_operators(norb: int):
+    """jl/hamiltonian.jl:404-426 quantum_operator_aabb: N, S^2 and S_z, with the real
+    S_y = 1/2 (a+_a a_b - a+_b a_a) of the reference and S^2 = S_x^2 - S_y^2 + S_z^2."""
+    N = []
+    for i in range(2 * norb):
+        N += fermion_operator([(i, 1), (i, 0)])
+    sx, sy, sz = [], [], []
+    for i in range(norb):
+        ia, ib = 2 * i, 2 * i + 1
+        ab, ba = fermion_operator([(ia, 1), (ib, 0)]), fermion_operator([(ib, 1), (ia, 0)])
+        sx = add_terms(sx, ab, ba, coeffs=[1.0, 0.5, 0.5])
+        sy = add_terms(sy, ab, ba, coeffs=[1.0, 0.5, -0.5])
+        sz = add_terms(sz, fermion_operator([(ia, 1), (ia, 0)]), fermion_operator([(ib, 1), (ib, 0)]),
+                       coeffs=[1.0, 0.5, -0.5])
+    N = _reduce(N, EPS2)
+    s2 = add_terms(multiply([sx, sx]), multiply([sy, sy]), multiply([sz, sz]), coeffs=[1.0, -1.0, 1.0])
+    return N, s2, sz
+
+
+def apply_constraint(h_terms, norb: int, nelec, constr_c):
+    """jl/hamiltonian.jl:429-435: H + c0 (N - na - nb)^2 + c1 S^2 + c2 S_z."""
+    N, s2, sz = spin_operators(norb)
+    shifted = add_terms(N, [(0, 0, -float(sum(nelec)))])
+    return add_terms(h_terms, multiply([shifted, shifted]), s2, sz, coeffs=[1.0, *map(float, constr_c)])
+
+
+def ising_terms(nq: int, J: float = 1.0, h: float = 0.5, pbc: bool = False):
+    """jl/hamiltonian.jl:438-461 ising_module: -J sum Z_i Z_{i+1} - h sum X_i on nq qubits."""
+    ops = [qubit_operator([(i, "Z"), (i + 1, "Z")], -J) for i in range(nq - 1)]
+    if pbc:
+        ops.append(qubit_operator([(nq - 1, "Z"), (0, "Z")], -J))
+    ops += [qubit_operator([(i, "X")], -h) for i in range(nq)]
+    return add_terms(*ops)
+
+
+def heisenberg_terms(nq: int, jx: float = 1.0, jy: float = 1.0, jz: float = 1.0, pbc: bool = False):
+    """jl/hamiltonian.jl:464-484 heisenberg_module: sum jx X_i X_j + jy Y_i Y_j + jz Z_i Z_j over
+    neighbouring qubits (j = i + 1, and nq-1 -- 0 with pbc)."""
+    bonds = [(i, i + 1) for i in range(nq - 1)] + ([(nq - 1, 0)] if pbc else [])
+    ops = []
+    for i, j in bonds:
+        ops += [qubit_operator([(i, "X"), (j, "X")], jx), qubit_operator([(i, "Y"), (j, "Y")], jy),
+                qubit_operator([(i, "Z"), (j, "Z")], jz)]
+    return add_terms(*ops)
+
+
+def term_arrays(terms):
+    """(x, z, c) qubit terms -> the (axs, bxs, azs, bzs, cs) arrays of a BinaryQubitAABB."""
+    arr = lambda f: np.array([f(t) for t in terms], dtype=np.uint32)  # noqa: E731
+    return (arr(lambda t: zip_even(t[0])), arr(lambda t: zip_odd(t[0])), arr(lambda t: zip_even(t[1])),
+            arr(lambda t: zip_odd(t[1])), np.array([t[2] for t in terms], dtype=np.float64))
+
+
 def ucc_generator(spin_orbital):
     """jl/ansatz.jl:230-250: G = T - T^dagger with T = a+_{vir..} a_{occ..}."""
     mid = len(spin_orbital) // 2

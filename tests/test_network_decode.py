@@ -2,12 +2,14 @@
 
 The device handle decodes a reference SVD network (jl/network.jl:158-339) into integrals and
 checks the decode against the network's own elements (src/include/core/math.hpp:37-124) on
-probe determinants.  Networks the decode cannot represent must come back NULL with an error
-instead of silently becoming a different operator: the S^2 penalty and the spin models of
-jl/hamiltonian.jl:405-484.  An (N - n)^2 particle-number penalty (apply_constraint,
-jl/hamiltonian.jl:429-435) is a spin-free two-body operator with 8-fold symmetric integrals,
-so H + mu (N - n)^2 decodes -- and the GPU test checks its sigma against the oracle, which
-evaluates the network itself.
+probe determinants.  A network the decode cannot represent must not silently become a
+different operator: a Hamiltonian-type network (it has the diagonal group) that fails the check
+-- the S^2 penalty and the spin models of jl/hamiltonian.jl:405-484, a term without its
+Jordan-Wigner string -- is kept as its Pauli groups (hipotf_network_kind 3, evaluated group by
+group: tests/test_gpu_network.py), and a pool network that fails comes back NULL with an error.
+An (N - n)^2 particle-number penalty (apply_constraint, jl/hamiltonian.jl:429-435) is a
+spin-free two-body operator with 8-fold symmetric integrals, so H + mu (N - n)^2 decodes (kind
+1) -- and the GPU test checks its sigma against the oracle, which evaluates the network itself.
 """
 import numpy as np
 import pytest
@@ -96,12 +98,16 @@ def _decode(n, net):
     return otf.HipOTF.from_network(n, net.flat)
 
 
+def _kind(h):
+    return L.load().hipotf_network_kind(h.ptr)
+
+
 def test_molecular_network_decodes():
-    assert _decode(6, O.hamiltonian_network(*integrals(6)[2:], 6)).ptr
+    assert _kind(_decode(6, O.hamiltonian_network(*integrals(6)[2:], 6))) == 1
 
 
 def test_number_penalty_decodes():
-    assert _decode(6, _number_penalty(6, 6, 0.7)).ptr
+    assert _kind(_decode(6, _number_penalty(6, 6, 0.7))) == 1
 
 
 @pytest.mark.parametrize("make", [
@@ -110,18 +116,16 @@ def test_number_penalty_decodes():
     lambda: _qubit_chain(12, zz=-1.0, x_field=-0.5),     # transverse-field Ising
     lambda: _qubit_chain(12, zz=1.0, xx=1.0),            # XXZ-type Heisenberg (no Y)
 ], ids=["s2_penalty", "ising_zz", "ising_tf", "heisenberg"])
-def test_non_two_body_networks_are_rejected(make):
-    with pytest.raises(L.HipOTFError, match="not supported"):
-        _decode(6, make())
+def test_non_two_body_networks_stay_pauli_groups(make):
+    assert _kind(_decode(6, make())) == 3
 
 
-def test_wrong_jordan_wigner_strings_are_rejected():
+def test_wrong_jordan_wigner_strings_stay_pauli_groups():
     # a hopping term without its Z string between the two alpha orbitals: a valid pattern
-    # whose elements disagree once an electron sits between them
+    # whose elements disagree with any integrals once an electron sits between them
     d = _ham_dict(6)
     _add(d, [((1 << 0) | (1 << 6), 0, 0.3)])
-    with pytest.raises(L.HipOTFError, match="spin-free two-body"):
-        _decode(6, _network(d))
+    assert _kind(_decode(6, _network(d))) == 3
 
 
 def test_pool_with_broken_generator_is_rejected():
