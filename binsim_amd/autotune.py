@@ -11,7 +11,9 @@ Tuned per precision (FP64 sigma of one vector; FP32 block sigma of ``k`` vectors
 * ``ab_kernel``: alpha-beta kernel family -- 1 source-lane (one beta source string per lane,
   destination alpha row in LDS), 2 register-descriptor (whole source row in LDS), 3 chunked,
   4 source-lane with FP32 vector pairs on the class-split form (the beta strings split by the
-  top orbital; one class's FP64 destination elements of both vectors in LDS);
+  top orbital; one class's FP64 destination elements of both vectors in LDS), 5 matrix-core
+  (per destination alpha row, the dense product of the integral block with the source rows on
+  the MFMA units, scattered into an FP64 LDS row; up to 16 orbitals, one vector per launch);
 * ``ab_split``: lanes per source string in the source-lane kernel, i.e. waves per CU
   (1: 512-thread workgroups, 8 waves per CU; 2: 1024-thread workgroups, 16 waves per CU);
 * ``ab_db``: one or two LDS copies of the replicated integral row (source-lane);
@@ -42,7 +44,7 @@ FIELDS = ("ab_kernel", "ab_split", "ab_db", "ab_nvs", "rs_passes", "deterministi
 FP64, FP32 = 0, 1
 DEFAULT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 
-_RANGES = {"ab_kernel": (0, 1, 2, 3, 4), "ab_split": (1, 2), "ab_db": (1, 2), "ab_nvs": (1, 2),
+_RANGES = {"ab_kernel": (0, 1, 2, 3, 4, 5), "ab_split": (1, 2), "ab_db": (1, 2), "ab_nvs": (1, 2),
            "rs_passes": (0, 1, 2, 4, 8), "deterministic": (0, 1), "rs_block": (0, 4, 8, 16),
            "sweep_mode": (0, 1, 2, 3)}
 
@@ -100,6 +102,7 @@ def candidates(precision: int, k: int) -> list[dict]:
                 out.append(validate(dict(ab_kernel=1, ab_split=s, ab_db=d, ab_nvs=v)))
     if precision == FP32 and k >= 2:
         out.append(validate(dict(ab_kernel=4)))
+    out.append(validate(dict(ab_kernel=5)))
     out.append(validate(dict(ab_kernel=2)))
     out.append(validate(dict(ab_kernel=3)))
     return out

@@ -644,7 +644,7 @@ int hipotf_set_tuning(int precision, const int32_t *v, int n) {
     API_TRY
     HIPOTF_REQUIRE(precision == 0 || precision == 1, "tuning: precision must be 0 or 1");
     HIPOTF_REQUIRE(v != nullptr && n == TUNING_FIELDS, "tuning: expected 8 values");
-    HIPOTF_REQUIRE(v[0] >= 0 && v[0] <= 4, "tuning: ab_kernel must be 0..4");
+    HIPOTF_REQUIRE(v[0] >= 0 && v[0] <= 5, "tuning: ab_kernel must be 0..5");
     HIPOTF_REQUIRE(v[1] == 1 || v[1] == 2, "tuning: ab_split must be 1 or 2");
     HIPOTF_REQUIRE(v[2] == 1 || v[2] == 2, "tuning: ab_db must be 1 or 2");
     HIPOTF_REQUIRE(v[3] == 1 || v[3] == 2, "tuning: ab_nvs must be 1 or 2");

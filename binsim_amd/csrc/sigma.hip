@@ -431,6 +431,7 @@ __global__ __launch_bounds__(1024) void k_ab_reg(const T *__restrict__ xt, T *__
 
 #include "ab_src.hpp"
 #include "ab_cls.hpp"
+#include "ab_mma.hpp"
 
 // ---------------------------------------------------------------------------------------
 // K5: sigma = E (s1 + s2T^T + E_core x~), FP64 output vectors
@@ -592,10 +593,11 @@ constexpr int ab_eptmax() { return (sizeof(T) == 8 && NVS >= 4) ? 4 : 8; }
 
 // alpha-beta kernel family (SigmaTuning::ab_kernel): 0 takes the first of class-split (FP32
 // vector pairs, ab_cls.hpp), source-lane, register, chunked that applies; 1..3 start the search
-// at source-lane, register, chunked; 4 is 1 with FP32 vector pairs on the class-split kernel
+// at source-lane, register, chunked; 4 is 1 with FP32 vector pairs on the class-split kernel;
+// 5 puts the matrix-core kernel (ab_mma.hpp, one vector per launch, n <= 16) first
 static int ab_kernel_pref(int prec) {
     const int p = tuning(prec).ab_kernel;
-    return p >= 0 && p <= 4 ? p : 0;
+    return p >= 0 && p <= 5 ? p : 0;
 }
 
 // bitwise reproducible sigma (the alpha-beta scatter runs wave by wave, ~15% slower); by
@@ -766,6 +768,35 @@ static bool try_ab_src_r(DevHam *H, const T *xt, T *s1, int nvst, int v0, const 
     }
 }
 
+// the matrix-core kernel (ab_mma.hpp) for vector v0: up to 16 orbitals (W rows n(n+1)/2 <= 9
+// tiles of 16, 16-bit strings), the FP64 destination row and KSP k-steps of W in LDS; not in
+// the deterministic mode (its LDS adds come from all waves at once)
+template <typename T>
+static bool try_ab_mma(DevHam *H, const T *xt, T *s1, int nvst, int v0, const T *eri, hipStream_t st) {
+    constexpr int prec = sizeof(T) == 4;
+    if (deterministic(prec)) return false;
+    DevBasis *B = H->basis;
+    const DevSpin &A = *B->pA, &Bs = *B->pB;
+    const int n = H->n;
+    constexpr int MT = 9, NWV = 8;
+    constexpr int KSP = sizeof(T) == 8 ? 9 : 17;  // FP64 at 16 orbitals: two passes
+    if (n > 16 || n < 1 || A.nsx + 1 > AB_SRC_CONN || Bs.N < 1) return false;
+    const i64 NB = Bs.N;
+    if ((u64)A.N * (u64)NB * (u64)nvst >= (1ull << 32)) return false;  // 32-bit element offsets
+    const size_t smem = ab_mma_lds<T, MT, KSP>(NB);
+    if (smem > std::min<size_t>(lds_budget(), 160 * 1024)) return false;
+    const i64 nblocks = (A.N + 7) / 8 * 8;
+    HIPOTF_REQUIRE(nblocks < (1ll << 31), "alpha-beta grid too large");
+    const T *dw = diag_w_table<T>(H, st);
+    const int nks = (A.nsx + 1 + 3) / 4;
+    auto kern = k_ab_mma<T, MT, KSP, NWV>;
+    HIPOTF_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(NWV * 64), smem, st, xt, s1, nvst, v0, A.sx.as<u32>(),
+                       Bs.strs.as<u64>(), eri, dw, ab_src_nw(n), n, A.N, (int)NB, A.nsx, nks);
+    HIPOTF_CHECK(hipGetLastError());
+    return true;
+}
+
 // the class-split FP32 kernel for vectors v0, v0 + 1 (ab_cls.hpp): 256-entry integral rows
 // (n <= 16), 1..8 beta electrons and holes, one class's FP64 destination elements of both
 // vectors in LDS; two launches (class T, class N)
@@ -862,8 +893,12 @@ static void launch_ab(DevHam *H, const T *xt, T *s1, int nv, const T *eri, hipSt
     auto fits = [&](int s) { return (size_t)NB * s * sizeof(T) <= budget; };
     int v0 = 0;
     const int pref = ab_kernel_pref(sizeof(T) == 4);
-    if (pref <= 1 || pref == 4) {
+    if (pref <= 1 || pref >= 4) {
         while (v0 < nv) {
+            if (pref == 5 && try_ab_mma<T>(H, xt, s1, nv, v0, eri, st)) {
+                v0 += 1;
+                continue;
+            }
             if constexpr (sizeof(T) == 4) {
                 if ((pref == 4 || pref == 0) && nv - v0 >= 2 && try_ab_cls(H, xt, s1, nv, v0, eri, st)) {
                     v0 += 2;

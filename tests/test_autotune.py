@@ -35,7 +35,7 @@ def test_defaults_and_per_precision_state(restore_tuning):
 def test_invalid_values_rejected_by_library_and_host(restore_tuning):
     lib = L.load()
     before = AT.get_config(AT.FP64)
-    for bad in ([5, 1, 1, 2, 1, 0, 8, 0], [0, 3, 1, 2, 1, 0, 8, 0], [0, 1, 0, 2, 1, 0, 8, 0],
+    for bad in ([6, 1, 1, 2, 1, 0, 8, 0], [0, 3, 1, 2, 1, 0, 8, 0], [0, 1, 0, 2, 1, 0, 8, 0],
                 [0, 1, 1, 3, 1, 0, 8, 0], [0, 1, 1, 2, 3, 0, 8, 0], [0, 1, 1, 2, 1, 2, 8, 0],
                 [0, 1, 1, 2, 1, 0, 2, 0], [0, 1, 1, 2, 1, 0, 8, 4]):
         v = np.array(bad, dtype=np.int32)
@@ -55,11 +55,11 @@ def test_invalid_values_rejected_by_library_and_host(restore_tuning):
 def test_candidates_cover_kernel_families_and_waves():
     c64 = AT.candidates(AT.FP64, 1)
     c32 = AT.candidates(AT.FP32, 4)
-    assert {c["ab_kernel"] for c in c64} == {1, 2, 3}
+    assert {c["ab_kernel"] for c in c64} == {1, 2, 3, 5}
     assert {c["ab_split"] for c in c64 if c["ab_kernel"] == 1} == {1, 2}
     assert {c["ab_nvs"] for c in c32 if c["ab_kernel"] == 1} == {1, 2}
-    assert 4 in {c["ab_kernel"] for c in c32}
-    assert len(c32) == 11 and len(c64) == 6
+    assert {4, 5} <= {c["ab_kernel"] for c in c32}
+    assert len(c32) == 12 and len(c64) == 7
 
 
 def test_json_store_round_trip(tmp_path):

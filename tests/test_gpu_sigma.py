@@ -91,6 +91,40 @@ def test_class_split_fp32_pairs(torch_cuda, n, na, nb, k):
         assert rel_err(host(y), r) < 2e-5
 
 
+# matrix-core alpha-beta kernel (ab_kernel 5): every n <= 16, including 16 orbitals with the
+# full 8-electron beta space (12,870 strings in the LDS row) and FP64 in two connection passes
+MMA = [(4, 2, 2), (6, 3, 3), (8, 3, 5), (10, 5, 2), (12, 6, 6), (12, 5, 6), (14, 3, 4), (16, 2, 2),
+       (16, 3, 2), (16, 1, 8), (16, 2, 8)]
+
+
+@pytest.mark.parametrize("n,na,nb", MMA)
+def test_mfma_alpha_beta(torch_cuda, n, na, nb):
+    """FP64 sigma (one vector) and FP32 block sigma (3 interleaved vectors) with the alpha-beta
+    part on the matrix-core kernel, against the oracle."""
+    from binsim_amd import autotune as AT
+    print(f"mfma alpha-beta n={n} na={na} nb={nb}", flush=True)
+    S = oracle_system(n, na, nb)
+    xs = [seeded_vector(S.dim, 700 + j) for j in range(3)]
+    refs = [S.hvec(x) for x in xs]
+    B, H, F = _dev(n, na, nb)
+    xd = [gpu_vec(torch_cuda, x) for x in xs]
+    yd = [torch_cuda.empty_like(v) for v in xd]
+    s64, s32 = AT.get_config(AT.FP64), AT.get_config(AT.FP32)
+    try:
+        AT.set_config(AT.FP64, dict(s64, ab_kernel=5, deterministic=0))
+        AT.set_config(AT.FP32, dict(s32, ab_kernel=5, deterministic=0))
+        F.hvec(xd[0], yd[0])
+        y = host(yd[0])
+        assert rel_err(y, refs[0]) < 1e-12
+        assert np.max(np.abs(y - refs[0])) < 1e-12 * np.max(np.abs(refs[0]))
+        F.hvec_block(xd, yd, precision=1)
+        for y, r in zip(yd, refs):
+            assert rel_err(host(y), r) < 2e-5
+    finally:
+        AT.set_config(AT.FP64, s64)
+        AT.set_config(AT.FP32, s32)
+
+
 @pytest.mark.parametrize("n,na,nb", [(17, 1, 6), (18, 1, 7)])
 def test_sigma_512_entry_forms_both_precisions(torch_cuda, n, na, nb):
     """The source-lane kernel's 512-entry forms with at most 7 beta electrons and 11 holes:
