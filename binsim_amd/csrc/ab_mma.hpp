@@ -31,7 +31,10 @@
 // D[4 v + l / 16][l % 16] for f64.
 #pragma once
 
-constexpr int AB_MMA_RANKS = 256 + 9 * 256;  // colex rank tables: low byte, (low count, high byte)
+constexpr int AB_MMA_RANKS = 256 + 9 * 256;
+// waves per workgroup (one workgroup per CU: the LDS holds the FP64 destination row): FP64 8
+// (224 VGPRs), FP32 12 (136 VGPRs; 8 waves: 4% slower)
+constexpr int AB_MMA_NWV64 = 8, AB_MMA_NWV32 = 12;  // colex rank tables: low byte, (low count, high byte)
 
 __device__ __forceinline__ u32 ab_mma_binom(int a, int b) {
     if (b < 0 || b > a) return 0u;
@@ -86,29 +89,46 @@ void k_ab_mma(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
     const i64 per = (NA + 7) / 8;
     const i64 Ia = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (Ia >= NA) return;
-    const int n2 = n * n, ntri = n * (n + 1) / 2;
+    const int n2 = n * n;
+    static_assert(MT == 9, "row classes: 1 diagonal, 2 + 2 one-byte and 4 cross-byte tiles");
 
-    for (int t = tid; t < NB; t += BD) S[t] = 0.0;
     for (int t = tid; t < nsxA; t += BD) conn[t] = sxA[(i64)t * NA + Ia];
-    // W rows: r = t for rows 0..n-1, then the pairs r < t in row-major order; a row's entry
-    // holds its orbital mask (low half) and the mask of the orbitals strictly between r and t
-    // (high half); padding rows hold 0 and never apply
+    // W rows by class of the orbital pair, so the target rank is formed with the fewest table
+    // reads: tile 0 the rows r = t (r < 16), tiles 1-2 the pairs inside the low byte (r < t < 8),
+    // tiles 3-4 inside the high byte (8 <= r < t), tiles 5-8 the pairs across (r < 8 <= t,
+    // row 80 + 8 r + t - 8).  A row's entry holds its orbital mask (low half) and the mask of
+    // the orbitals strictly between r and t (high half); rows past n, and padding, hold 0 and
+    // never apply.
     for (int i = tid; i < MT * 16; i += BD) {
-        u32 v = 0;
-        if (i < n) {
-            v = 1u << i;
-        } else if (i < ntri) {
-            int r = 0, e = i - n;
-            while (e >= n - 1 - r) {
-                e -= n - 1 - r;
-                ++r;
+        int r = -1, t = -1;
+        if (i < 16) {
+            r = t = i;
+        } else if (i < 80) {
+            const int lo = i < 48 ? 0 : 8, e0 = i - (i < 48 ? 16 : 48);
+            int e = e0, a = 0;
+            while (a < 7 && e >= 7 - a) {
+                e -= 7 - a;
+                ++a;
             }
-            const int t = r + 1 + e;
-            v = (1u << r) | (1u << t) | ((((1u << t) - (2u << r)) & 0xffffu) << 16);
+            if (e0 < 28) {
+                r = lo + a;
+                t = r + 1 + e;
+            }
+        } else {
+            r = (i - 80) >> 3;
+            t = 8 + ((i - 80) & 7);
         }
+        u32 v = 0;
+        if (r >= 0 && t < n)
+            v = r == t ? 1u << r : (1u << r) | (1u << t) | ((((1u << t) - (2u << r)) & 0xffffu) << 16);
         rinfo[i] = v;
     }
-    // colex ranks rank(s) = rk[s & 255] + rk[256 + 256 * popc(s & 255) + (s >> 8)]
+    // colex ranks rank(s) = rk[s & 255] + rk[256 + 256 * popc(s & 255) + (s >> 8)], from the
+    // binomials C(a, b), a < 16, b <= 16, staged at the start of the (not yet used) destination
+    // row and W area
+    u32 *bn = reinterpret_cast<u32 *>(S);
+    for (int e = tid; e < 16 * 17; e += BD) bn[e] = ab_mma_binom(e / 17, e % 17);
+    __syncthreads();
     for (int e = tid; e < AB_MMA_RANKS; e += BD) {
         const int hi = e >= 256;
         const int c0 = hi ? (e - 256) >> 8 : 0, bits = hi ? (e - 256) & 255 : e, base = hi ? 8 : 0;
@@ -116,13 +136,29 @@ void k_ab_mma(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
         int j = c0;
         for (int q = 0; q < 8; ++q)
             if ((bits >> q) & 1) {
-                v += ab_mma_binom(base + q, j + 1);
+                v += j + 1 <= 16 ? bn[(base + q) * 17 + j + 1] : 0u;
                 ++j;
             }
         rk[e] = (unsigned short)v;
     }
     __syncthreads();
+    for (int t = tid; t < NB; t += BD) S[t] = 0.0;
+    __syncthreads();
 
+    // FP64: this lane's W rows (result element v of row tile m) stay in registers; the FP32
+    // kernel (12 waves, 168 VGPRs) reads them from LDS
+    constexpr bool HOIST = sizeof(T) == 8;
+    u32 rl[HOIST ? MT : 1][4];
+    if constexpr (HOIST) {
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) rl[m][v] = rinfo[m * 16 + Ops::row(g, v)];
+    }
+    auto row_info = [&](int m, int v) -> u32 {
+        if constexpr (HOIST) return rl[m % MT][v];
+        return rinfo[m * 16 + Ops::row(g, v)];
+    };
     const int ntiles = (NB + 15) >> 4;
     const int npass = (nks + KSP - 1) / KSP;
     for (int pass = 0; pass < npass; ++pass) {
@@ -132,8 +168,8 @@ void k_ab_mma(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
             const int l = e & 63, m = (e >> 6) % MT, s = (e >> 6) / MT;
             const int i = m * 16 + (l & 15), c = 4 * (s0 + s) + (l >> 4);
             T v = T(0);
-            if (i < ntri && c <= nsxA) {
-                const u32 om = rinfo[i] & 0xffffu;
+            const u32 om = rinfo[i] & 0xffffu;
+            if (om != 0u && c <= nsxA) {
                 const int r = __builtin_ctz(om), t = 31 - __builtin_clz(om);
                 if (c < nsxA) {
                     const u32 d = conn[c];
@@ -178,32 +214,65 @@ void k_ab_mma(const T *__restrict__ xt, T *__restrict__ s1, int nvst, int v0, co
             V4 acc[MT];
 #pragma unroll
             for (int m = 0; m < MT; ++m) acc[m] = V4{T(0), T(0), T(0), T(0)};
+            // A operands of the next k-step are read while this one's MFMAs run
+            T a0[MT], a1[MT];
 #pragma unroll
-            for (int s = 0; s < KSP; ++s)
+            for (int m = 0; m < MT; ++m) a0[m] = Wl[m * 64 + lane];
 #pragma unroll
-                for (int m = 0; m < MT; ++m) acc[m] = Ops::mma(Wl[(s * MT + m) * 64 + lane], b[s], acc[m]);
+            for (int s = 0; s < KSP; ++s) {
+                T(&ac)[MT] = (s & 1) ? a1 : a0;
+                T(&an)[MT] = (s & 1) ? a0 : a1;
+                if (s + 1 < KSP) {
+#pragma unroll
+                    for (int m = 0; m < MT; ++m) an[m] = Wl[((s + 1) * MT + m) * 64 + lane];
+                }
+#pragma unroll
+                for (int m = 0; m < MT; ++m) acc[m] = Ops::mma(ac[m], b[s], acc[m]);
+            }
+            // scatter.  Colex rank of a target Ib = Jb ^ om: rank = rk[lo(Ib)] + rk[256 +
+            // 256 popc(lo(Ib)) + hi(Ib)]; a pair inside one byte leaves the other byte's term
+            // as it is for Jb.  Two row tiles at a time, every target rank is read first
+            // (branch-free, so the reads are issued together), then the adds; the sign goes
+            // into the value.
             const int Jb = nt * 16 + col;
-            if (Jb < NB) {
-                double dsum = 0.0;
+            const bool live = Jb < NB;
+            const u32 jlo = jb & 255u, jbase = 256u + ((u32)__popc(jlo) << 8);
+            const u32 RL = rk[jlo], RH = rk[jbase + (jb >> 8)];
+            double dsum = 0.0;
 #pragma unroll
-                for (int m = 0; m < MT; ++m)
+            for (int v = 0; v < 4; ++v)
+                if (jb & row_info(0, v)) dsum += (double)acc[0][v];
+#pragma unroll
+            for (int m0 = 1; m0 < MT; m0 += 2) {
+                u32 tg[2][4];
+#pragma unroll
+                for (int mm = 0; mm < 2; ++mm)
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
-                        const int i = m * 16 + Ops::row(g, v);
-                        const u32 ri = rinfo[i];
-                        const u32 om = ri & 0xffffu, x = jb & om;
-                        const double y = (double)acc[m][v];
-                        if (i < n) {
-                            if (x) dsum += y;
-                        } else if (x != 0u && x != om) {
-                            const u32 ib = jb ^ om, lo = ib & 255u;
-                            const u32 rank = (u32)rk[lo] + (u32)rk[256 + (__popc(lo) << 8) + (ib >> 8)];
-                            const u32 sg = (u32)(__popc(jb & (ri >> 16)) & 1) << 31;
-                            atomicAdd(&S[rank], flip_sign(y, sg));
+                        const int m = m0 + mm;
+                        const u32 ri = row_info(m, v), om = ri & 0xffffu, x = jb & om;
+                        const u32 ib = jb ^ om;
+                        u32 rank;
+                        if (m <= 2) {
+                            rank = (u32)rk[ib & 255u] + RH;
+                        } else if (m <= 4) {
+                            rank = RL + (u32)rk[jbase + (ib >> 8)];
+                        } else {
+                            const u32 l2 = ib & 255u;
+                            rank = (u32)rk[l2] + (u32)rk[256u + ((u32)__popc(l2) << 8) + (ib >> 8)];
                         }
+                        acc[m][v] = flip_sign(acc[m][v], (u32)(__popc(jb & (ri >> 16)) & 1) << 31);
+                        u32 tv = (live && x != 0u && x != om) ? rank : 0xffffffffu;
+                        asm volatile("" : "+v"(tv));  // test the value itself at the add
+                        tg[mm][v] = tv;
                     }
-                atomicAdd(&S[Jb], dsum);
+#pragma unroll
+                for (int mm = 0; mm < 2; ++mm)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        if (tg[mm][v] != 0xffffffffu) atomicAdd(&S[tg[mm][v]], (double)acc[m0 + mm][v]);
             }
+            if (live) atomicAdd(&S[Jb], dsum);
         }
         __syncthreads();  // W is refilled by the next pass
     }

@@ -778,7 +778,7 @@ static bool try_ab_mma(DevHam *H, const T *xt, T *s1, int nvst, int v0, const T 
     DevBasis *B = H->basis;
     const DevSpin &A = *B->pA, &Bs = *B->pB;
     const int n = H->n;
-    constexpr int MT = 9, NWV = 8;
+    constexpr int MT = 9, NWV = sizeof(T) == 8 ? AB_MMA_NWV64 : AB_MMA_NWV32;
     constexpr int KSP = sizeof(T) == 8 ? 9 : 17;  // FP64 at 16 orbitals: two passes
     if (n > 16 || n < 1 || A.nsx + 1 > AB_SRC_CONN || Bs.N < 1) return false;
     const i64 NB = Bs.N;
