@@ -39,14 +39,13 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
 
-# HBM bytes of one FP64 sigma build at 16 orbitals 8a8b, from two rocprofv3 --pmc runs of
-# `bench.py --pmc-unit` (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, summed over the
-# dispatches of one sigma), each scaled by what the same counter reads for a plain copy kernel
-# of 1.325 GB in the same run (profiles/sigma_fp64_16o_pmc.md)
-SIGMA_TRAFFIC_16O = 266.4e9
-# average launch time of the FP64 alpha-beta kernel in one 16-orbital sigma, from the kernel
-# trace of `bench.py --steps 3 --warmup 1 --quick --no-cpu` (profiles/sigma_fp64_16o_kernel_trace.md)
-AB_SRC_MS_16O = 144.7
+PEAK_MFMA_F64_TFLOPS = 78.6  # dense FP64 MFMA
+
+# HBM bytes per launch of the FP64 matrix-core alpha-beta kernel at 16 orbitals 8a8b (one launch
+# per destination class, two per sigma), from two rocprofv3 --pmc runs of `bench.py --pmc-unit`
+# (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the same counter reads
+# for a plain copy kernel of 1.325 GB in the same run (profiles/ab_mma_16o_pmc.md)
+AB_MMA_TRAFFIC_16O = 50.77e9
 
 
 def log(*a):
@@ -465,20 +464,30 @@ def main():
     torch.cuda.synchronize()
     ms32 = e0.elapsed_time(e1) / 4
     del xs, ys
-    # the dominant kernel of one FP64 sigma is the alpha-beta source-lane kernel (~70% of the
-    # time, profiles/sigma_fp64_16o_kernel_trace.md); it is bound by LDS traffic and issue, not
-    # HBM.  Its LDS bytes per launch: one 8-byte integral gather per (determinant, alpha
-    # connection, beta slot) plus k diagonal gathers, the replicated r <= t integral row (32
-    # copies) written once per (alpha row, chunk of 512 source strings, connection), and a
-    # read + write per FP64 scatter add (one per slot and the diagonal, per chunk)
-    from math import comb
-    Nn = comb(n, na)
-    sa, conn = na * (n - na), na * (n - na) + 1
-    ntri = n * (n + 1) // 2
-    lds_bytes = 8 * (B.dim * conn * (sa + nb) + Nn * ((Nn + 511) // 512) * conn * ntri * 32
-                     + 2 * B.dim * (sa + 1))
-    lds_peak = 128 * 256 * 2.4e9 / 1e9  # GB/s: 128 B/clk/CU x 256 CUs x 2.4 GHz
-    ab_ms = AB_SRC_MS_16O
+    # the dominant kernel of one FP64 sigma: the alpha-beta part on the matrix cores
+    # (k_ab_mma, two launches per sigma), timed live with HIP events on the library's stream
+    # (hipotf_ab_timing: events around each sigma's alpha-beta launches)
+    from binsim_amd import lib as L
+    import ctypes
+    lib = L.load()
+    tms, tsec, tlaunch = ctypes.c_double(), ctypes.c_int64(), ctypes.c_int64()
+    L.check(lib.hipotf_ab_timing(1, None, None, None) == 0, "hipotf_ab_timing")
+    for _ in range(args.steps):
+        F.hvec(x, y)
+    torch.cuda.synchronize()
+    L.check(lib.hipotf_ab_timing(0, ctypes.byref(tms), ctypes.byref(tsec), ctypes.byref(tlaunch)) == 0,
+            "hipotf_ab_timing")
+    ab_section_ms = tms.value / max(1, tsec.value)
+    ab_launches = tlaunch.value / max(1, tsec.value)
+    # algorithmic FLOPs of the alpha-beta part: the dense product of the n(n+1)/2 x (s_a + 1)
+    # integral block with the (s_a + 1) source rows, for every determinant
+    ntri, nca = n * (n + 1) // 2, na * (n - na) + 1
+    ab_flops = 2.0 * B.dim * nca * ntri
+    per_launch_ms = ab_section_ms / ab_launches if ab_launches else ab_section_ms
+    per_launch_flops = ab_flops / ab_launches if ab_launches else ab_flops
+    ab_tf = per_launch_flops / (per_launch_ms * 1e-3) / 1e12
+    log(f"alpha-beta: {ab_section_ms:.2f} ms per sigma, {ab_launches:.0f} matrix-core launches, "
+        f"{ab_tf:.1f} TFLOP/s")
     result = {
         "metric": "FCI solve s + sigma GB/s, mixed vs FP64 at |dE|<1e-9 Ha; VQE adjoint grads/s",
         "value": round(gbs, 2),
@@ -488,27 +497,25 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded integrals and vectors)",
         "config": {"workload": "FCI sigma build H x, 16 orbitals 8a8b (165,636,900 determinants)",
                    "norb": n, "nalpha": na, "nbeta": nb, "dim": B.dim, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4),
-                     "traffic": SIGMA_TRAFFIC_16O,
-                     "achieved_is": "the algorithmic row-stream model B_sigma = 8 dim (2 + s_a + d_a) "
-                                    "divided by the time of one sigma, not measured HBM bytes",
-                     "algorithmic_bytes_per_launch": bs,
-                     "traffic_source": "profiles/sigma_fp64_16o_pmc.md (FETCH_SIZE + WRITE_SIZE of "
-                                       "one sigma, each calibrated against a plain copy)",
-                     "measured_hbm_GBps": round(SIGMA_TRAFFIC_16O / (ms * 1e-3) / 1e9, 1),
-                     "measured_hbm_frac": round(SIGMA_TRAFFIC_16O / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
-                     "kernel": "one sigma build (hvec_cuda): k_gauge_in, 2 row SpMMs (k_rbspmm, "
-                               "source-range passes), k_ab_src, k_finalize on the default stream, "
-                               "timed with HIP events as one unit",
-                     "dominant_kernel": {
-                         "name": "k_ab_src<double, 1, 8, 8, 32, 1, 1, 256, true> (alpha-beta)",
-                         "avg_ms": ab_ms, "bound": "lds",
-                         "lds_bytes_per_launch": lds_bytes,
-                         "lds_GBps": round(lds_bytes / (ab_ms * 1e-3) / 1e9, 1),
-                         "lds_peak_GBps": lds_peak,
-                         "lds_frac": round(lds_bytes / (ab_ms * 1e-3) / 1e9 / lds_peak, 4),
-                         "source": "profiles/sigma_fp64_16o_kernel_trace.md, profiles/sigma_fp64_16o_pmc.md"}},
+        "roofline": {"bound": "mfma", "achieved": round(ab_tf, 2), "peak": PEAK_MFMA_F64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(ab_tf / PEAK_MFMA_F64_TFLOPS, 4),
+                     "traffic": AB_MMA_TRAFFIC_16O,
+                     "kernel": "k_ab_mma<double, 9, 17, 8, true>: alpha-beta part of the FP64 sigma "
+                               "on the FP64 MFMA units, one launch per destination class",
+                     "achieved_is": "dense-product FLOPs 2 dim (s_a + 1) n(n+1)/2 per sigma divided by "
+                                    "the launches, over the average launch time from HIP events on the "
+                                    "library stream (hipotf_ab_timing)",
+                     "avg_launch_ms": round(per_launch_ms, 3), "launches_per_sigma": ab_launches,
+                     "flops_per_launch": per_launch_flops,
+                     "traffic_source": "profiles/ab_mma_16o_pmc.md (FETCH_SIZE + WRITE_SIZE per "
+                                       "dispatch, each calibrated against a plain copy)",
+                     "sigma_hbm_model": {
+                         "algorithmic_bytes": bs, "GBps": round(gbs, 2),
+                         "frac_of_8TBps": round(gbs / PEAK_HBM_GBS, 4),
+                         "note": "the row-stream model B_sigma = 8 dim (2 + s_a + d_a) of the whole "
+                                 "sigma (the value above) counts one HBM read per source element and "
+                                 "connection; the kernels reuse source rows from L2 and LDS, so it can "
+                                 "exceed the HBM peak"}},
         "sigma_fp32_block4_ms_per_vector": round(ms32, 3),
         "tuning": tuning,
         "setup_s": round(setup_s, 3), "wall_s_timed": round(wall, 3),

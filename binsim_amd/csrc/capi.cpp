@@ -683,6 +683,28 @@ int hipotf_get_tuning(int precision, int32_t *v, int n) {
     API_CATCH(1)
 }
 
+// alpha-beta section timing: enable 1 zeroes the totals and starts, 0 stops, -1 leaves the state;
+// returns (through non-null pointers) the totals since the last start
+int hipotf_ab_timing(int enable, double *ms, int64_t *sections, int64_t *mma_launches) {
+    API_TRY
+    HIPOTF_REQUIRE(enable >= -1 && enable <= 1, "ab timing: enable must be -1, 0 or 1");
+    AbTiming &t = ab_timing();
+    if (enable == 1) {
+        if (!t.e0) HIPOTF_CHECK(hipEventCreate(&t.e0));
+        if (!t.e1) HIPOTF_CHECK(hipEventCreate(&t.e1));
+        t.ms = 0.0;
+        t.sections = t.mma_launches = 0;
+        t.on = true;
+    } else if (enable == 0) {
+        t.on = false;
+    }
+    if (ms) *ms = t.ms;
+    if (sections) *sections = t.sections;
+    if (mma_launches) *mma_launches = t.mma_launches;
+    return 0;
+    API_CATCH(1)
+}
+
 int64_t hipotf_num_generators(void *net) {
     API_TRY
     DevNet *N = as_net(net);

@@ -230,6 +230,15 @@ struct SigmaTuning {
                             // at most one row and half of the next per block in LDS
 };
 SigmaTuning &tuning(int precision);  // 0: FP64 kernels, 1: FP32 kernels
+// timing of the sigma builds' alpha-beta sections (hipotf_ab_timing): while on, HIP events on
+// the sigma stream around each section, synchronized and accumulated
+struct AbTiming {
+    bool on = false;
+    double ms = 0.0;
+    long long sections = 0, mma_launches = 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+AbTiming &ab_timing();
 constexpr int TUNING_FIELDS = 8;
 
 // sigma: dst_v = H src_v for nv vectors; prec 0 = float64 work, 1 = float32 work; at most

@@ -600,6 +600,11 @@ static int ab_kernel_pref(int prec) {
     return p >= 0 && p <= 5 ? p : 0;
 }
 
+AbTiming &ab_timing() {
+    static AbTiming t;
+    return t;
+}
+
 // bitwise reproducible sigma (the alpha-beta scatter runs wave by wave, ~15% slower); by
 // default its FP summation order depends on scheduling.  The initial value comes from the
 // environment (HIPOTF_DETERMINISTIC=1).
@@ -768,9 +773,10 @@ static bool try_ab_src_r(DevHam *H, const T *xt, T *s1, int nvst, int v0, const 
     }
 }
 
-// the matrix-core kernel (ab_mma.hpp) for vector v0: up to 16 orbitals (W rows n(n+1)/2 <= 9
-// tiles of 16, 16-bit strings), the FP64 destination row and KSP k-steps of W in LDS; not in
-// the deterministic mode (its LDS adds come from all waves at once)
+// the matrix-core kernel (ab_mma.hpp) for vector v0: up to 16 orbitals (16-bit strings), W and
+// the FP64 destination elements in LDS -- FP32 the whole row, FP64 one class of destinations
+// per launch (split by the top orbital); not in the deterministic mode (its LDS adds come from
+// all waves at once)
 template <typename T>
 static bool try_ab_mma(DevHam *H, const T *xt, T *s1, int nvst, int v0, const T *eri, hipStream_t st) {
     constexpr int prec = sizeof(T) == 4;
@@ -778,22 +784,33 @@ static bool try_ab_mma(DevHam *H, const T *xt, T *s1, int nvst, int v0, const T 
     DevBasis *B = H->basis;
     const DevSpin &A = *B->pA, &Bs = *B->pB;
     const int n = H->n;
-    constexpr int MT = 9, NWV = sizeof(T) == 8 ? AB_MMA_NWV64 : AB_MMA_NWV32;
-    constexpr int KSP = sizeof(T) == 8 ? 9 : 17;  // FP64 at 16 orbitals: two passes
-    if (n > 16 || n < 1 || A.nsx + 1 > AB_SRC_CONN || Bs.N < 1) return false;
+    constexpr bool CLS = sizeof(T) == 8;
+    constexpr int MT = 9, KSP = 17, NWV = CLS ? AB_MMA_NWV64 : AB_MMA_NWV32;
+    if (n > 16 || n < 1 || A.nsx + 1 > 4 * KSP || A.nsx + 1 > AB_SRC_CONN || Bs.N < 1) return false;
     const i64 NB = Bs.N;
     if ((u64)A.N * (u64)NB * (u64)nvst >= (1ull << 32)) return false;  // 32-bit element offsets
-    const size_t smem = ab_mma_lds<T, MT, KSP>(NB);
+    // class N: the beta strings without orbital n - 1, the first C(n - 1, k) in colex order
+    i64 NN = NB;
+    if (CLS) {
+        NN = 1;
+        for (int i = 0; i < Bs.k; ++i) NN = NN * (n - 1 - i) / (i + 1);
+        if (Bs.k > n - 1) NN = 0;
+    }
+    const size_t smem = ab_mma_lds<T, MT, KSP>(CLS ? std::max(NN, NB - NN) : NB);
     if (smem > std::min<size_t>(lds_budget(), 160 * 1024)) return false;
     const i64 nblocks = (A.N + 7) / 8 * 8;
     HIPOTF_REQUIRE(nblocks < (1ll << 31), "alpha-beta grid too large");
     const T *dw = diag_w_table<T>(H, st);
     const int nks = (A.nsx + 1 + 3) / 4;
-    auto kern = k_ab_mma<T, MT, KSP, NWV>;
+    auto kern = k_ab_mma<T, MT, KSP, NWV, CLS>;
     HIPOTF_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-    hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(NWV * 64), smem, st, xt, s1, nvst, v0, A.sx.as<u32>(),
-                       Bs.strs.as<u64>(), eri, dw, ab_src_nw(n), n, A.N, (int)NB, A.nsx, nks);
-    HIPOTF_CHECK(hipGetLastError());
+    for (int cls = 0; cls < (CLS ? 2 : 1); ++cls) {
+        if (CLS && (cls ? NB - NN : NN) == 0) continue;
+        hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(NWV * 64), smem, st, xt, s1, nvst, v0, A.sx.as<u32>(),
+                           Bs.strs.as<u64>(), eri, dw, ab_src_nw(n), n, A.N, (int)NB, A.nsx, nks, cls, (int)NN);
+        HIPOTF_CHECK(hipGetLastError());
+        ++ab_timing().mma_launches;
+    }
     return true;
 }
 
@@ -957,7 +974,17 @@ static void sigma_group(DevHam *H, const SrcPack &src, const DstPack &dst, hipSt
     launch_rowspmm<T>(xt, s1, *H->ptA, CA, NA, NB * NV, st);
     launch_rowspmm<T>(xtT, s2T, *H->ptB, CB, NB, NA * NV, st);
     const T *eri = f32 ? (const T *)H->eri32.p : (const T *)H->eri64.p;
+    AbTiming &tm = ab_timing();
+    if (tm.on) HIPOTF_CHECK(hipEventRecord(tm.e0, st));
     launch_ab<T>(H, xt, s1, NV, eri, st);
+    if (tm.on) {
+        HIPOTF_CHECK(hipEventRecord(tm.e1, st));
+        HIPOTF_CHECK(hipEventSynchronize(tm.e1));
+        float ms = 0.f;
+        HIPOTF_CHECK(hipEventElapsedTime(&ms, tm.e0, tm.e1));
+        tm.ms += ms;
+        ++tm.sections;
+    }
     hipLaunchKernelGGL((k_finalize<T, NV>), g2, dim3(256), 0, st, s1, s2T, xt, H->ecore, dst,
                        Bs.strs.as<u64>(), B->gaugeM.as<u64>(), NA, NB);
     HIPOTF_CHECK(hipGetLastError());

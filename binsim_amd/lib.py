@@ -54,6 +54,7 @@ SIGNATURES = {
     "hipotf_vqe_energy_grad": (C.c_int, [_vp, _vp, _vp, C.c_int64, _i64p, _dp, _vp, _vp, _dp, _dp]),
     "hipotf_vqe_energy": (C.c_int, [_vp, _vp, _vp, C.c_int64, _i64p, _dp, _vp, _vp, _dp]),
     "hipotf_network_kind": (C.c_int, [_vp]),
+    "hipotf_ab_timing": (C.c_int, [C.c_int, _dp, _i64p, _i64p]),
     "hipotf_davidson_f64": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
                                       C.c_int, C.c_int, _vp, _dp, _vp, _dp]),
     "build_ritz_and_residual_f64": (C.c_double, [C.c_int64, C.c_int, _dp, C.c_double, _dpp, _dpp, _vp]),

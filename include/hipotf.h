@@ -106,7 +106,8 @@ int64_t hipotf_num_generators(void *net);
 int hipotf_prepare(void *basis, void *net);
 /* launch choices of the sigma kernels, process-wide (no reference counterpart; the
  * reference's tiles are compile-time, src/include/ham/otf.hpp:6-9 and cuda/common.cuh:7-12):
- * v[8] = {alpha-beta kernel 0 auto | 1 source-lane | 2 register | 3 chunked, threads per
+ * v[8] = {alpha-beta kernel 0 auto | 1 source-lane | 2 register | 3 chunked | 4 source-lane
+ * with FP32 vector pairs on the class-split kernel | 5 matrix-core (16 orbitals or fewer), threads per
  * source string 1|2, integral-row buffers 1|2, FP32 vectors per alpha-beta launch 1|2,
  * row-SpMM source-range passes 1|2|4|8 (0: as many as keep a column tile's gathered slice
  * within an XCD's L2), deterministic 0|1, same-spin destination rows per
@@ -119,6 +120,14 @@ int hipotf_prepare(void *basis, void *net);
  * choice as JSON.  Return 0 on success. */
 int hipotf_set_tuning(int precision, const int32_t *v, int n);
 int hipotf_get_tuning(int precision, int32_t *v, int n);
+/* Timing of the alpha-beta part of the sigma builds (no counterpart in the reference; for
+ * benchmarks and roofline figures).  enable = 1 zeroes the totals and starts, 0 stops, -1 only
+ * reads.  While on, every sigma build records HIP events on its stream around its alpha-beta
+ * launches and waits for them (one synchronization per build; not for use under graph
+ * capture).  Writes through the non-null pointers: milliseconds summed over the timed sections,
+ * the number of sections, and how many of the launches in them were the matrix-core kernel
+ * (ab_kernel 5).  Process-wide, not thread-safe.  Return 0 on success. */
+int hipotf_ab_timing(int enable, double *ms, int64_t *sections, int64_t *mma_launches);
 /* hvec_cuda without device allocation.  No counterpart in the reference: it is for callers that
  * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
  * that forbid hipMalloc).  dst = H src in FP64 as hvec_cuda computes it, with the four sigma
