@@ -591,10 +591,14 @@ static void launch_ab_ept(DevHam *H, const T *xt, T *s1, int nvst, int v0, const
 template <typename T, int NVS>
 constexpr int ab_eptmax() { return (sizeof(T) == 8 && NVS >= 4) ? 4 : 8; }
 
-// alpha-beta kernel family (SigmaTuning::ab_kernel): 0 takes the first of class-split (FP32
-// vector pairs, ab_cls.hpp), source-lane, register, chunked that applies; 1..3 start the search
-// at source-lane, register, chunked; 4 is 1 with FP32 vector pairs on the class-split kernel;
-// 5 puts the matrix-core kernel (ab_mma.hpp, one vector per launch, n <= 16) first
+// alpha-beta kernel family (SigmaTuning::ab_kernel): 0 takes the first of matrix-core (14 to 16
+// orbitals), class-split (FP32 vector pairs, ab_cls.hpp), source-lane, register, chunked that
+// applies; 1..3 start the search at source-lane, register, chunked; 4 is 1 with FP32 vector
+// pairs on the class-split kernel; 5 puts the matrix-core kernel (ab_mma.hpp, one vector per
+// launch, n <= 16) first.  The matrix-core kernel's default range is where it was measured
+// faster: at 12 orbitals its FP64 sigma is 8% slower (0.66 vs 0.61 ms), at 14 and 16 it is 1.4
+// and 1.6 times faster.
+static bool ab_mma_first(int pref, int n) { return pref == 5 || (pref == 0 && n >= 14 && n <= 16); }
 static int ab_kernel_pref(int prec) {
     const int p = tuning(prec).ab_kernel;
     return p >= 0 && p <= 5 ? p : 0;
@@ -912,7 +916,7 @@ static void launch_ab(DevHam *H, const T *xt, T *s1, int nv, const T *eri, hipSt
     const int pref = ab_kernel_pref(sizeof(T) == 4);
     if (pref <= 1 || pref >= 4) {
         while (v0 < nv) {
-            if (pref == 5 && try_ab_mma<T>(H, xt, s1, nv, v0, eri, st)) {
+            if (ab_mma_first(pref, H->n) && try_ab_mma<T>(H, xt, s1, nv, v0, eri, st)) {
                 v0 += 1;
                 continue;
             }

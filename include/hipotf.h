@@ -106,8 +106,9 @@ int64_t hipotf_num_generators(void *net);
 int hipotf_prepare(void *basis, void *net);
 /* launch choices of the sigma kernels, process-wide (no reference counterpart; the
  * reference's tiles are compile-time, src/include/ham/otf.hpp:6-9 and cuda/common.cuh:7-12):
- * v[8] = {alpha-beta kernel 0 auto | 1 source-lane | 2 register | 3 chunked | 4 source-lane
- * with FP32 vector pairs on the class-split kernel | 5 matrix-core (16 orbitals or fewer), threads per
+ * v[8] = {alpha-beta kernel 0 auto (matrix-core at 14-16 orbitals, else class-split FP32 pairs,
+ * source-lane, register, chunked) | 1 source-lane | 2 register | 3 chunked | 4 source-lane with
+ * FP32 vector pairs on the class-split kernel | 5 matrix-core (16 orbitals or fewer), threads per
  * source string 1|2, integral-row buffers 1|2, FP32 vectors per alpha-beta launch 1|2,
  * row-SpMM source-range passes 1|2|4|8 (0: as many as keep a column tile's gathered slice
  * within an XCD's L2), deterministic 0|1, same-spin destination rows per
