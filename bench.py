@@ -45,7 +45,7 @@ PEAK_MFMA_F64_TFLOPS = 78.6  # dense FP64 MFMA
 # per destination class, two per sigma), from two rocprofv3 --pmc runs of `bench.py --pmc-unit`
 # (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the same counter reads
 # for a plain copy kernel of 1.325 GB in the same run (profiles/ab_mma_16o_pmc.md)
-AB_MMA_TRAFFIC_16O = 50.77e9
+AB_MMA_TRAFFIC_16O = 50.75e9
 
 
 def log(*a):
@@ -500,7 +500,7 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(ab_tf, 2), "peak": PEAK_MFMA_F64_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(ab_tf / PEAK_MFMA_F64_TFLOPS, 4),
                      "traffic": AB_MMA_TRAFFIC_16O,
-                     "kernel": "k_ab_mma<double, 9, 17, 8, true>: alpha-beta part of the FP64 sigma "
+                     "kernel": "k_ab_mma<double, 9, 17, 16, true>: alpha-beta part of the FP64 sigma "
                                "on the FP64 MFMA units, one launch per destination class",
                      "achieved_is": "dense-product FLOPs 2 dim (s_a + 1) n(n+1)/2 per sigma divided by "
                                     "the launches, over the average launch time from HIP events on the "

@@ -800,7 +800,7 @@ static bool try_ab_mma(DevHam *H, const T *xt, T *s1, int nvst, int v0, const T 
         for (int i = 0; i < Bs.k; ++i) NN = NN * (n - 1 - i) / (i + 1);
         if (Bs.k > n - 1) NN = 0;
     }
-    const size_t smem = ab_mma_lds<T, MT, KSP>(CLS ? std::max(NN, NB - NN) : NB);
+    const size_t smem = ab_mma_lds<T, MT, KSP, NWV>(CLS ? std::max(NN, NB - NN) : NB);
     if (smem > std::min<size_t>(lds_budget(), 160 * 1024)) return false;
     const i64 nblocks = (A.N + 7) / 8 * 8;
     HIPOTF_REQUIRE(nblocks < (1ll << 31), "alpha-beta grid too large");
