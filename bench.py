@@ -266,6 +266,66 @@ def batchgrad_time(torch, n, reps=3):
     return out
 
 
+def rank4_time(torch, n, nsample=48):
+    """tvec / backtran / batch expm (4 interleaved vectors) on a sample of generators and batch
+    tran over the whole pool at n orbitals, n/2 + n/2 electrons: milliseconds per call (HIP
+    events) and algorithmic GB/s from each generator's pair count (tvec: 8 B read + 8 B written
+    per element of a pair plus the zero fill of dst; backtran: l, r read and written, b written;
+    batch expm: nvec values of both rows read and written)."""
+    from math import comb
+    from binsim_amd.ansatz import ucc_spin_orbitals
+    B, H, P, F = build(n, n // 2, n // 2, pool=True)
+    na = nb = n // 2
+    sos = ucc_spin_orbitals(n, na, nb)
+    ng = len(sos)
+    NA, NB = comb(n, na), comb(n, nb)
+
+    def pairs(so):
+        half = len(so) // 2
+        m = {0: [0, 0], 1: [0, 0]}  # spin -> [creators, annihilators]
+        for i, pos in enumerate(so):
+            m[pos & 1][0 if i < half else 1] += 1
+        pa = comb(n - sum(m[0]), na - m[0][1]) if sum(m[0]) else NA
+        pb = comb(n - sum(m[1]), nb - m[1][1]) if sum(m[1]) else NB
+        return pa * pb
+
+    gens = list(range(0, ng, max(1, ng // nsample)))
+    npairs = sum(pairs(sos[k]) for k in gens)
+    g = torch.Generator(device="cuda").manual_seed(91 + n)
+    l, r, x = (torch.rand(B.dim, dtype=torch.float64, device="cuda", generator=g) - 0.5 for _ in range(3))
+    y, b = torch.empty_like(x), torch.empty_like(x)
+    m = torch.rand(B.dim * 4, dtype=torch.float64, device="cuda", generator=g) - 0.5
+    out = {"rank4": f"{n} orbitals {na}a{nb}b", "generators": ng, "sampled": len(gens)}
+
+    def timed(fn):
+        fn(gens[0])
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for k in gens:
+            fn(k)
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / len(gens)
+
+    for name, fn, bpp, fill in (("tvec", lambda k: F.tvec(k, x, y), 32, 8),
+                                ("backtran", lambda k: F.backtran(k, 0.01, l, r, b), 80, 8),
+                                ("batchexpm_4", lambda k: F.batchexpm(k, 0.01, m, 4, 4), 128, 0)):
+        ms = timed(fn)
+        gbs = (npairs / len(gens) * bpp + fill * B.dim) / (ms * 1e-3) / 1e9
+        out[name] = {"ms_per_call": round(ms, 3), "algorithmic_GBps": round(gbs, 1)}
+        log(f"rank4 n={n} {name}: {ms:.3f} ms per generator, {gbs:.0f} GB/s")
+    t = torch.empty(ng, dtype=torch.float64, device="cuda")
+    F.batchtran(l, r, t)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    F.batchtran(l, r, t)
+    torch.cuda.synchronize()
+    out["batchtran_seconds"] = round(time.perf_counter() - t0, 4)
+    log(f"rank4 n={n} batchtran over {ng} generators: {out['batchtran_seconds']} s")
+    return out
+
+
 def cpu_baseline(budget_s=10.0, extras=True, dav_iters=200):
     """The reference's OpenMP CPU path (oracle/_ref) on the 12-orbital system: the sigma,
     repeated (the reported value), then (extras) the Davidson ground state from HF to 1e-8 and
@@ -379,6 +439,8 @@ def main():
     ap.add_argument("--davidson-only", type=int, default=None, metavar="MAXITER",
                     help="only the 16-orbital ground state to 1e-6: plain FP64, then mixed (block 1, "
                          "maxspace 32), each stopped after MAXITER iterations (600: to convergence)")
+    ap.add_argument("--rank4", type=int, choices=(12, 16), default=None, metavar="N",
+                    help="time tvec / backtran / batch expm / batch tran at N orbitals")
     ap.add_argument("--batchgrad", type=int, choices=(12, 16), default=None, metavar="N",
                     help="only time the pool gradient (batchgrad_cuda) at N orbitals: fused pass vs "
                          "one launch per generator")
@@ -416,6 +478,9 @@ def main():
         return
     if args.batchgrad:
         print(json.dumps(batchgrad_time(torch, args.batchgrad)))
+        return
+    if args.rank4:
+        print(json.dumps(rank4_time(torch, args.rank4)))
         return
     if args.davidson_only:
         print(json.dumps(fci16_ground(torch, args.davidson_only)))

@@ -801,6 +801,39 @@ double backgrad_cuda_2d(void *basis, void *net, int64_t idx, double theta, doubl
     return backgrad_cuda(basis, net, idx, theta, lp, rp);
 }
 
+// ---- real-time / batched pool operators (libotf_native only in the reference:
+// src/include/op/declare.hpp:76-130; the same arguments with device pointers) ------------------
+void tvec_cuda(void *basis, void *net, int64_t idx, const double *src, double *dst) {
+    API_TRY
+    DevBasis *B = as_basis(basis);
+    rot_tvec(as_net(net)->pool_for(B), idx, src, dst, S0());
+    API_CATCH_VOID
+}
+
+void backtran_cuda(void *basis, void *net, int64_t idx, double theta, double *lp, double *rp, double *bp) {
+    API_TRY
+    DevBasis *B = as_basis(basis);
+    rot_backtran(as_net(net)->pool_for(B), idx, theta, lp, rp, bp, S0());
+    API_CATCH_VOID
+}
+
+void batchexpm_cuda(void *basis, void *net, int64_t idx, double theta, double *matrix, int64_t ld,
+                    int64_t nvec) {
+    API_TRY
+    DevBasis *B = as_basis(basis);
+    rot_batch_expm(as_net(net)->pool_for(B), idx, theta, matrix, ld, nvec, S0());
+    API_CATCH_VOID
+}
+
+void batchtran_cuda(void *basis, void *net, const double *lp, const double *rp, double *trans) {
+    API_TRY
+    DevBasis *B = as_basis(basis);
+    DevPool *P = as_net(net)->pool_for(B);
+    HIPOTF_REQUIRE(lp && rp && trans, "NULL argument");
+    batch_tran(P, lp, rp, trans, S0());
+    API_CATCH_VOID
+}
+
 // grads[g] = grad(g, thetas[g], lp, rp) for every generator (src/include/cuda/batchgrad.cuh:106-378);
 // thetas / grads are device arrays.  One gradient-only pass over the sweeps' fused segments:
 // the generators sharing an alpha pattern (or a row kind) read their rows once per segment

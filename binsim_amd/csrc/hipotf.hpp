@@ -281,6 +281,7 @@ struct DevPool {
     DevBuf segdesc, segpart;       // fused sweeps: per-position descriptors, per-block partials
     DevBuf tl, tr;                 // fused sweeps: beta-major copies of the two state vectors
     DevBuf vqe_res;                // fused VQE objective: per-parameter results + norms (grown, kept)
+    DevBuf zeros;                  // batch tran: a zero angle per generator
 };
 
 // forward expm sweep and/or adjoint backgrad sweep over positions idxs[0..n) with fused
@@ -293,6 +294,8 @@ void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, doub
 // segments; thetas and grads are device arrays
 void fused_grads(DevPool *P, const double *thetas, const double *lp, const double *rp, double *grads,
                  hipStream_t st);
+// trans[g] = the gradient of generator g at theta = 0 (op/batchtran.hpp:517-528), device arrays
+void batch_tran(DevPool *P, const double *lp, const double *rp, double *trans, hipStream_t st);
 
 DevPool *make_dev_pool(DevBasis *b, i64 nops, const int32_t *offsets, const int32_t *orbs);
 
@@ -308,6 +311,12 @@ void gennet_hvec(DevGenNet *D, DevBasis *B, const double *x, double *y, hipStrea
 void gennet_diag(DevGenNet *D, DevBasis *B, double *diag, hipStream_t st);               // accumulates
 
 void rot_expm(DevPool *P, i64 idx, double theta, double *vec, hipStream_t st);
+// dst = G_idx src (dst zeroed first)
+void rot_tvec(DevPool *P, i64 idx, const double *src, double *dst, hipStream_t st);
+// lp, rp <- exp(theta G_idx) lp, rp; bp = G_idx lp (bp zeroed first)
+void rot_backtran(DevPool *P, i64 idx, double theta, double *lp, double *rp, double *bp, hipStream_t st);
+// exp(theta G_idx) on columns 0..nvec-1 of the det-major matrix m[i * ld + k]
+void rot_batch_expm(DevPool *P, i64 idx, double theta, double *m, i64 ld, i64 nvec, hipStream_t st);
 // grad: returns device pointer to the scalar result (written asynchronously)
 void rot_grad(DevPool *P, i64 idx, double theta, const double *lp, const double *rp, double *dres,
               hipStream_t st);

@@ -137,6 +137,69 @@ __global__ __launch_bounds__(256) void k_backgrad(PairView v, GenDev g, double c
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
+// tvec (src/include/op/tvec.hpp:373-445, tvec_update of core/math.hpp:164-170): dst = G src,
+// per pair dst_u = -t src_w, dst_w = t src_u (dst zeroed first by k_fill)
+__global__ __launch_bounds__(256) void k_tvec(PairView v, GenDev g, const double *__restrict__ src,
+                                              double *__restrict__ dst) {
+    const i64 tot = v.na * v.nb;
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (i64)gridDim.x * blockDim.x) {
+        i64 u, w;
+        u64 ua, ub;
+        pair_at(v, i, u, w, ua, ub);
+        const double t = jw_sign(g, ua, ub) ? -1.0 : 1.0;
+        dst[u] = -t * src[w];
+        dst[w] = t * src[u];
+    }
+}
+
+// backtran (op/backtran.hpp:383-452, backtran_update of math.hpp:217-225): l and r rotated as
+// expm rotates, then b = G l of the rotated l (b zeroed first)
+__global__ __launch_bounds__(256) void k_backtran(PairView v, GenDev g, double c, double s,
+                                                  double *__restrict__ l, double *__restrict__ r,
+                                                  double *__restrict__ b) {
+    const i64 tot = v.na * v.nb;
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (i64)gridDim.x * blockDim.x) {
+        i64 u, w;
+        u64 ua, ub;
+        pair_at(v, i, u, w, ua, ub);
+        const bool neg = jw_sign(g, ua, ub);
+        const double t = neg ? -1.0 : 1.0, ts = neg ? -s : s;
+        const double lu = l[u], lw = l[w], ru = r[u], rw = r[w];
+        const double nlu = c * lu - ts * lw, nlw = ts * lu + c * lw;
+        l[u] = nlu;
+        l[w] = nlw;
+        r[u] = c * ru - ts * rw;
+        r[w] = ts * ru + c * rw;
+        b[u] = -t * nlw;
+        b[w] = t * nlu;
+    }
+}
+
+// batch expm (op/batchexpm.hpp:376-444, expm_batch_update_matrix of math.hpp:227-252): the
+// rotation on columns 0..nvec-1 of a det-major matrix m[i * ld + k]; a pair's two rows are
+// ld-strided, the columns of a row contiguous
+__global__ __launch_bounds__(256) void k_batch_expm(PairView v, GenDev g, double c, double s,
+                                                    double *__restrict__ m, i64 ld, int nvec) {
+    const i64 tot = v.na * v.nb;
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (i64)gridDim.x * blockDim.x) {
+        i64 u, w;
+        u64 ua, ub;
+        pair_at(v, i, u, w, ua, ub);
+        const double ts = jw_sign(g, ua, ub) ? -s : s;
+        double *mu = m + u * ld, *mw = m + w * ld;
+        for (int k = 0; k < nvec; ++k) {
+            const double xu = mu[k], xw = mw[k];
+            mu[k] = c * xu - ts * xw;
+            mw[k] = ts * xu + c * xw;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill(double *__restrict__ x, i64 n, double val) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
+        x[i] = val;
+}
+
 // sum of partial[0..n) into *out (+ optional scale) by one block, fixed order -> deterministic
 __global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__ partial, int n,
                                                       double scale, double *__restrict__ out) {
@@ -292,6 +355,53 @@ void rot_expm(DevPool *P, i64 idx, double theta, double *vec, hipStream_t st) {
     HIPOTF_REQUIRE(vec != nullptr, "NULL vector");
     hipLaunchKernelGGL(k_expm, dim3(grid_for(tot)), dim3(256), 0, st, v, g, std::cos(theta),
                        std::sin(theta), vec);
+    HIPOTF_CHECK(hipGetLastError());
+}
+
+static void fill(double *x, i64 n, double val, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(256), 0, st, x, n, val);
+    HIPOTF_CHECK(hipGetLastError());
+}
+
+void rot_tvec(DevPool *P, i64 idx, const double *src, double *dst, hipStream_t st) {
+    PairView v;
+    GenDev g;
+    views(P, idx, v, g);
+    HIPOTF_REQUIRE(src != nullptr && dst != nullptr, "NULL vector");
+    HIPOTF_REQUIRE(src != dst, "tvec: src and dst must be different vectors");
+    fill(dst, P->basis->host.dim, 0.0, st);
+    const i64 tot = v.na * v.nb;
+    if (tot == 0) return;
+    hipLaunchKernelGGL(k_tvec, dim3(grid_for(tot)), dim3(256), 0, st, v, g, src, dst);
+    HIPOTF_CHECK(hipGetLastError());
+}
+
+void rot_backtran(DevPool *P, i64 idx, double theta, double *lp, double *rp, double *bp, hipStream_t st) {
+    PairView v;
+    GenDev g;
+    views(P, idx, v, g);
+    HIPOTF_REQUIRE(lp && rp && bp, "NULL vector");
+    HIPOTF_REQUIRE(bp != lp && bp != rp && lp != rp, "backtran: lp, rp and bp must be different vectors");
+    fill(bp, P->basis->host.dim, 0.0, st);
+    const i64 tot = v.na * v.nb;
+    if (tot == 0) return;
+    hipLaunchKernelGGL(k_backtran, dim3(grid_for(tot)), dim3(256), 0, st, v, g, std::cos(theta),
+                       std::sin(theta), lp, rp, bp);
+    HIPOTF_CHECK(hipGetLastError());
+}
+
+void rot_batch_expm(DevPool *P, i64 idx, double theta, double *m, i64 ld, i64 nvec, hipStream_t st) {
+    PairView v;
+    GenDev g;
+    views(P, idx, v, g);
+    HIPOTF_REQUIRE(m != nullptr, "NULL matrix");
+    HIPOTF_REQUIRE(ld >= 1 && nvec >= 0 && nvec <= ld && nvec < (1ll << 31),
+                   "batch expm: need 0 <= nvec <= ld");
+    const i64 tot = v.na * v.nb;
+    if (tot == 0 || nvec == 0) return;
+    hipLaunchKernelGGL(k_batch_expm, dim3(grid_for(tot)), dim3(256), 0, st, v, g, std::cos(theta),
+                       std::sin(theta), m, ld, (int)nvec);
     HIPOTF_CHECK(hipGetLastError());
 }
 
@@ -832,6 +942,15 @@ static void sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, con
 void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, double *psi, double *lam,
                   bool forward, bool backward, double *dres, hipStream_t st) {
     sweeps(P, n, idxs, thetas, nullptr, psi, lam, forward, backward, false, dres, st);
+}
+
+// batch tran (op/batchtran.hpp:517-528): tran_update summed per generator, i.e. the gradient
+// of every generator at theta = 0 (the gradient-only pass on a zero angle vector)
+void batch_tran(DevPool *P, const double *lp, const double *rp, double *trans, hipStream_t st) {
+    const i64 n = (i64)P->gens.size();
+    P->zeros.ensure((size_t)std::max<i64>(n, 1) * sizeof(double));
+    fill(P->zeros.as<double>(), n, 0.0, st);
+    fused_grads(P, P->zeros.as<double>(), lp, rp, trans, st);
 }
 
 void fused_grads(DevPool *P, const double *thetas, const double *lp, const double *rp, double *grads,

@@ -40,6 +40,10 @@ SIGNATURES = {
     "grad_cuda": (C.c_double, [_vp, _vp, C.c_int64, C.c_double, _vp, _vp]),
     "backgrad_cuda": (C.c_double, [_vp, _vp, C.c_int64, C.c_double, _vp, _vp]),
     "batchgrad_cuda": (None, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "tvec_cuda": (None, [_vp, _vp, C.c_int64, _vp, _vp]),
+    "backtran_cuda": (None, [_vp, _vp, C.c_int64, C.c_double, _vp, _vp, _vp]),
+    "batchexpm_cuda": (None, [_vp, _vp, C.c_int64, C.c_double, _vp, C.c_int64, C.c_int64]),
+    "batchtran_cuda": (None, [_vp, _vp, _vp, _vp, _vp]),
     "expm_cuda_2d": (None, [_vp, _vp, C.c_int64, C.c_double, _vp]),
     "grad_cuda_2d": (C.c_double, [_vp, _vp, C.c_int64, C.c_double, _vp, _vp]),
     "backgrad_cuda_2d": (C.c_double, [_vp, _vp, C.c_int64, C.c_double, _vp, _vp]),

@@ -176,6 +176,25 @@ class HipOTF_Functions:
         L.check(not L.last_error(), "backgrad_cuda")
         return g
 
+    def tvec(self, idx: int, lv, rv):
+        """rv = G_idx lv (jl/network.jl:572-574 f_tvec)"""
+        L.call_void("tvec_cuda", self.basis.ptr, self.pool.ptr, int(idx), _ptr(lv), _ptr(rv))
+
+    def backtran(self, idx: int, theta: float, lv, rv, tlv):
+        """lv, rv <- exp(theta G_idx) lv, rv; tlv = G_idx lv (f_backtran)"""
+        L.call_void("backtran_cuda", self.basis.ptr, self.pool.ptr, int(idx), float(theta), _ptr(lv),
+                    _ptr(rv), _ptr(tlv))
+
+    def batchexpm(self, idx: int, theta: float, mat, N: int, j: int):
+        """exp(theta G_idx) on the first j of the N interleaved vectors of mat (det-major,
+        mat[i * N + k]) (f_batchexpm)"""
+        L.call_void("batchexpm_cuda", self.basis.ptr, self.pool.ptr, int(idx), float(theta), _ptr(mat),
+                    int(N), int(j))
+
+    def batchtran(self, lv, rv, trans):
+        """trans[k] = -<lv|G_k|rv> for every pool generator (f_batchtran)"""
+        L.call_void("batchtran_cuda", self.basis.ptr, self.pool.ptr, _ptr(lv), _ptr(rv), _ptr(trans))
+
     def batchgrad(self, lv, rv, grads, thetas):
         L.call_void("batchgrad_cuda", self.basis.ptr, self.pool.ptr, _ptr(thetas), _ptr(lv),
                     _ptr(rv), _ptr(grads))

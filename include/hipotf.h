@@ -82,6 +82,24 @@ double backgrad_cuda(void *basis, void *net, int64_t idx, double theta, double *
 /* batchgrad_cuda (cuotf.cu:111-122): grads[g] = grad(g, thetas[g], lp, rp); device arrays */
 void batchgrad_cuda(void *basis, void *net, const double *thetas, const double *lp,
                     const double *rp, double *grads);
+/* The pool operators the reference has on the CPU only (libotf_native,
+ * src/include/op/declare.hpp:76-130; jl/network.jl:572-596 binds them as tvec, backtran,
+ * batchexpm, batchtran), here on device vectors with a device basis / pool handle.
+ * tvec_contract_otf (op/tvec.hpp:373-445): dst = G_idx src; dst overwritten (zero outside the
+ * generator's pairs); src != dst */
+void tvec_cuda(void *basis, void *net, int64_t idx, const double *src, double *dst);
+/* backtran_contract_otf (op/backtran.hpp:383-452): lp, rp <- exp(theta G_idx) lp, rp, then
+ * bp = G_idx lp (bp overwritten); three different vectors */
+void backtran_cuda(void *basis, void *net, int64_t idx, double theta, double *lp, double *rp,
+                   double *bp);
+/* batch_expm_contract_otf (op/batchexpm.hpp:376-444): exp(theta G_idx) applied to columns
+ * 0..nvec-1 of the det-major matrix m[i * ld + k] (dim rows of ld entries; nvec <= ld).  The
+ * reference takes int ld / num_vecs; the widths here are 64-bit. */
+void batchexpm_cuda(void *basis, void *net, int64_t idx, double theta, double *matrix,
+                    int64_t ld, int64_t nvec);
+/* batch_tran_contract_otf (op/batchtran.hpp:517-528): trans[g] = sum over the pairs of G_g of
+ * tran_update (math.hpp:181-185) = -<lp| G_g |rp>, for every generator; device arrays */
+void batchtran_cuda(void *basis, void *net, const double *lp, const double *rp, double *trans);
 void expm_cuda_2d(void *basis, void *net, int64_t idx, double theta, double *vec); /* cuotf.cu:124-133 */
 double grad_cuda_2d(void *basis, void *net, int64_t idx, double theta, const double *lp,
                     const double *rp);                                            /* cuotf.cu:135-146 */

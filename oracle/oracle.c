@@ -8,6 +8,11 @@
  *                         formulas of core/math.hpp:132-215; each pair (src,dst), dst = src^x,
  *                         is visited once, from the source with the larger determinant index
  *                         (expm.hpp:104-131, 317-318)
+ *   tvec / backtran       op/tvec.hpp:373-445, op/backtran.hpp:383-452 (tvec_update,
+ *                         backtran_update: math.hpp:164-225; the output vector zeroed first)
+ *   batch expm / tran     op/batchexpm.hpp:376-444 (expm_batch_update_matrix: the pair's rows
+ *                         of a det-major matrix with leading dimension ld), op/batchtran.hpp:
+ *                         517-528 (tran_update summed per generator)
  *   libdiag               src/include/diag/davidson.hpp:31-272
  */
 #include "oracle.h"
@@ -167,9 +172,11 @@ void oracle_diags(int n, int na, int nb, const oracle_net *net, double *diag) {
     basis_free(&b);
 }
 
-/* mode 0 expm, 1 grad, 2 backgrad */
+/* mode 0 expm, 1 grad, 2 backgrad, 3 tvec (v -> l), 4 backtran (l, r; b = v), 5 tran (cl, cr),
+ * 6 batch expm (v = matrix with leading dimension ld, nvec vectors) */
 static double rot(int n, int na, int nb, const oracle_net *net, int64_t gi, double theta, double *v,
-                  double *l, double *r, const double *cl, const double *cr, int mode) {
+                  double *l, double *r, const double *cl, const double *cr, int mode, int64_t ld,
+                  int nvec) {
     basis_t b;
     basis_init(&b, n, na, nb);
     group_t *G = groups_of(net);
@@ -202,6 +209,27 @@ static double rot(int n, int na, int nb, const oracle_net *net, int64_t gi, doub
             } else if (mode == 1) { /* grad_update, cd = -sin, co = cos */
                 const double vd = -sth * vt * vt, fw = cth * vt;
                 res += cl[si] * (cr[si] * vd + cr[di] * fw) + cl[di] * (cr[di] * vd - cr[si] * fw);
+            } else if (mode == 3) { /* tvec_update: dst_s = -vt src_d, dst_d = vt src_s */
+                l[si] = -vt * v[di];
+                l[di] = vt * v[si];
+            } else if (mode == 4) { /* backtran_update: expm on l and r, then b = tvec(l) */
+                const double vd = 1.0 + (cth - 1.0) * vt * vt, fw = sth * vt;
+                const double lvi = l[si], lvj = l[di], rvi = r[si], rvj = r[di];
+                l[si] = lvi * vd - lvj * fw;
+                l[di] = lvj * vd + lvi * fw;
+                r[si] = rvi * vd - rvj * fw;
+                r[di] = rvj * vd + rvi * fw;
+                v[si] = -vt * l[di];
+                v[di] = vt * l[si];
+            } else if (mode == 5) { /* tran_update: (l_s vt) r_d - l_d vt r_s */
+                res += cl[si] * vt * cr[di] - cl[di] * vt * cr[si];
+            } else if (mode == 6) { /* expm_batch_update_matrix */
+                const double vd = 1.0 + (cth - 1.0) * vt * vt, fw = sth * vt;
+                for (int k = 0; k < nvec; ++k) {
+                    const double vi = v[si * ld + k], vj = v[di * ld + k];
+                    v[si * ld + k] = vi * vd - vj * fw;
+                    v[di * ld + k] = vj * vd + vi * fw;
+                }
             } else { /* backgrad_update: ecd = cos-1, eco = -sin, gcd = -sin, gco = cos */
                 const double evd = 1.0 + (cth - 1.0) * vt * vt, efw = -sth * vt;
                 const double gvd = -sth * vt * vt, gfw = cth * vt;
@@ -223,15 +251,38 @@ static double rot(int n, int na, int nb, const oracle_net *net, int64_t gi, doub
 }
 
 void oracle_expm(int n, int na, int nb, const oracle_net *net, int64_t g, double theta, double *v) {
-    rot(n, na, nb, net, g, theta, v, NULL, NULL, NULL, NULL, 0);
+    rot(n, na, nb, net, g, theta, v, NULL, NULL, NULL, NULL, 0, 0, 0);
 }
 double oracle_grad(int n, int na, int nb, const oracle_net *net, int64_t g, double theta,
                    const double *l, const double *r) {
-    return rot(n, na, nb, net, g, theta, NULL, NULL, NULL, l, r, 1);
+    return rot(n, na, nb, net, g, theta, NULL, NULL, NULL, l, r, 1, 0, 0);
 }
 double oracle_backgrad(int n, int na, int nb, const oracle_net *net, int64_t g, double theta,
                        double *l, double *r) {
-    return rot(n, na, nb, net, g, theta, NULL, l, r, NULL, NULL, 2);
+    return rot(n, na, nb, net, g, theta, NULL, l, r, NULL, NULL, 2, 0, 0);
+}
+static int64_t dim_of(int n, int na, int nb) {
+    basis_t b;
+    basis_init(&b, n, na, nb);
+    const int64_t d = b.NA * b.NB;
+    basis_free(&b);
+    return d;
+}
+void oracle_tvec(int n, int na, int nb, const oracle_net *net, int64_t g, const double *src, double *dst) {
+    memset(dst, 0, sizeof(double) * (size_t)dim_of(n, na, nb));
+    rot(n, na, nb, net, g, 0.0, (double *)src, dst, NULL, NULL, NULL, 3, 0, 0);
+}
+void oracle_backtran(int n, int na, int nb, const oracle_net *net, int64_t g, double theta, double *l,
+                     double *r, double *b) {
+    memset(b, 0, sizeof(double) * (size_t)dim_of(n, na, nb));
+    rot(n, na, nb, net, g, theta, b, l, r, NULL, NULL, 4, 0, 0);
+}
+double oracle_tran(int n, int na, int nb, const oracle_net *net, int64_t g, const double *l, const double *r) {
+    return rot(n, na, nb, net, g, 0.0, NULL, NULL, NULL, l, r, 5, 0, 0);
+}
+void oracle_batch_expm(int n, int na, int nb, const oracle_net *net, int64_t g, double theta, double *m,
+                       int64_t ld, int nvec) {
+    rot(n, na, nb, net, g, theta, m, NULL, NULL, NULL, NULL, 6, ld, nvec);
 }
 
 /* ---- libdiag --------------------------------------------------------------------------- */

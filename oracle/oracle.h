@@ -33,6 +33,17 @@ double oracle_grad(int n, int na, int nb, const oracle_net *net, int64_t g, doub
                    const double *l, const double *r);
 double oracle_backgrad(int n, int na, int nb, const oracle_net *net, int64_t g, double theta,
                        double *l, double *r);
+/* dst = G_g src (dst zeroed first): op/tvec.hpp:373-445 */
+void oracle_tvec(int n, int na, int nb, const oracle_net *net, int64_t g, const double *src, double *dst);
+/* l, r <- exp(theta G_g) l, r; b = G_g l (b zeroed first): op/backtran.hpp:383-452 */
+void oracle_backtran(int n, int na, int nb, const oracle_net *net, int64_t g, double theta, double *l,
+                     double *r, double *b);
+/* sum over the pairs of G_g of tran_update (math.hpp:181-185): one entry of batch_tran
+ * (op/batchtran.hpp:517-528) */
+double oracle_tran(int n, int na, int nb, const oracle_net *net, int64_t g, const double *l, const double *r);
+/* exp(theta G_g) on vectors 0..nvec-1 of m[i * ld + k]: op/batchexpm.hpp:376-444 */
+void oracle_batch_expm(int n, int na, int nb, const oracle_net *net, int64_t g, double theta, double *m,
+                       int64_t ld, int nvec);
 /* libdiag (diag/davidson.hpp:31-272) */
 double oracle_ritz_and_residual(int64_t N, int dim, const double *c, double e, const double *const *V,
                                 const double *const *AV, double *axt);

@@ -55,6 +55,12 @@ def lib():
         L.oracle_grad.restype = C.c_double
         L.oracle_backgrad.argtypes = [C.c_int, C.c_int, C.c_int, P, C.c_int64, C.c_double, _dp, _dp]
         L.oracle_backgrad.restype = C.c_double
+        L.oracle_tvec.argtypes = [C.c_int, C.c_int, C.c_int, P, C.c_int64, _dp, _dp]
+        L.oracle_backtran.argtypes = [C.c_int, C.c_int, C.c_int, P, C.c_int64, C.c_double, _dp, _dp, _dp]
+        L.oracle_tran.argtypes = [C.c_int, C.c_int, C.c_int, P, C.c_int64, _dp, _dp]
+        L.oracle_tran.restype = C.c_double
+        L.oracle_batch_expm.argtypes = [C.c_int, C.c_int, C.c_int, P, C.c_int64, C.c_double, _dp,
+                                        C.c_int64, C.c_int]
         pp = C.POINTER(_dp)
         L.oracle_ritz_and_residual.argtypes = [C.c_int64, C.c_int, _dp, C.c_double, pp, pp, _dp]
         L.oracle_ritz_and_residual.restype = C.c_double
@@ -128,6 +134,32 @@ class System:
 
     def backgrad(self, g, theta, l, r):
         return lib().oracle_backgrad(self.n, self.na, self.nb, self.pool.ref, g, theta, _p(l), _p(r))
+
+    def tvec(self, g, src):
+        """G_g src (op/tvec.hpp:373-445)"""
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        dst = np.empty_like(src)
+        lib().oracle_tvec(self.n, self.na, self.nb, self.pool.ref, g, _p(src), _p(dst))
+        return dst
+
+    def backtran(self, g, theta, l, r):
+        """l, r rotated in place by exp(theta G_g); returns G_g l (op/backtran.hpp:383-452)"""
+        b = np.empty_like(l)
+        lib().oracle_backtran(self.n, self.na, self.nb, self.pool.ref, g, theta, _p(l), _p(r), _p(b))
+        return b
+
+    def batch_tran(self, ngen, l, r):
+        """[sum of tran_update over the pairs of G_k] for k < ngen (op/batchtran.hpp:517-528)"""
+        l = np.ascontiguousarray(l, dtype=np.float64)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        return np.array([lib().oracle_tran(self.n, self.na, self.nb, self.pool.ref, k, _p(l), _p(r))
+                         for k in range(ngen)])
+
+    def batch_expm(self, g, theta, m, ld, nvec):
+        """exp(theta G_g) on columns 0..nvec-1 of the det-major matrix m (row stride ld),
+        in place (op/batchexpm.hpp:376-444)"""
+        assert m.dtype == np.float64 and m.flags.c_contiguous and m.size >= self.dim * ld
+        lib().oracle_batch_expm(self.n, self.na, self.nb, self.pool.ref, g, theta, _p(m), ld, nvec)
 
 
 def hf_vector(sysm: System):

@@ -68,6 +68,15 @@ class Reference:
         o.backgrad_contract_otf_f64.restype = C.c_double
         o.backgrad_contract_otf_f64.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_double,
                                                 _dp, _dp]
+        o.tvec_contract_otf_f64.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, _dp, _dp]
+        o.tvec_contract_otf_f64.restype = None
+        o.backtran_contract_otf_f64.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_double, _dp, _dp, _dp]
+        o.backtran_contract_otf_f64.restype = None
+        o.batch_expm_contract_otf_f64.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_double, _dp,
+                                                  C.c_int, C.c_int]
+        o.batch_expm_contract_otf_f64.restype = None
+        o.batch_tran_contract_otf_f64.argtypes = [C.c_void_p, C.c_void_p, _dp, _dp, _dp]
+        o.batch_tran_contract_otf_f64.restype = None
         pp = C.POINTER(_dp)
         d.build_ritz_and_residual_f64.restype = C.c_double
         d.build_ritz_and_residual_f64.argtypes = [C.c_int64, C.c_int, _dp, C.c_double, pp, pp, _dp]
@@ -145,6 +154,18 @@ class Reference:
     def backgrad(self, basis, net, idx, theta, lv, rv):
         return self.otf.backgrad_contract_otf_f64(basis, net, idx, theta, _ptr(lv, _dp),
                                                   _ptr(rv, _dp))
+
+    def tvec(self, basis, net, idx, src, dst):
+        self.otf.tvec_contract_otf_f64(basis, net, idx, _ptr(src, _dp), _ptr(dst, _dp))
+
+    def backtran(self, basis, net, idx, theta, lv, rv, bv):
+        self.otf.backtran_contract_otf_f64(basis, net, idx, theta, _ptr(lv, _dp), _ptr(rv, _dp), _ptr(bv, _dp))
+
+    def batch_expm(self, basis, net, idx, theta, m, ld, nvec):
+        self.otf.batch_expm_contract_otf_f64(basis, net, idx, theta, _ptr(m, _dp), ld, nvec)
+
+    def batch_tran(self, basis, net, lv, rv, trans):
+        self.otf.batch_tran_contract_otf_f64(basis, net, _ptr(lv, _dp), _ptr(rv, _dp), _ptr(trans, _dp))
 
     # -- davidson.hpp -------------------------------------------------------------------
     @staticmethod
