@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(HERE, "libhipotf.so")
-SOURCES = ["basis.cpp", "ham.cpp", "capi.cpp", "sigma.hip", "rot.hip", "diag.hip", "davidson.cpp", "network.hip"]
+SOURCES = ["basis.cpp", "ham.cpp", "capi.cpp", "sigma.hip", "rot.hip", "diag.hip", "davidson.cpp", "network.hip", "tables.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HIPOTF_ARCH", "gfx950")

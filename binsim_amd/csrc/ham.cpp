@@ -134,6 +134,12 @@ const RowBlockTable &row_blocks(SameSpinTable &T, int rb) {
     HIPOTF_REQUIRE(rb == 4 || rb == 8 || rb == 16, "row block size must be 4, 8 or 16");
     const int slot = rb == 4 ? 0 : (rb == 8 ? 1 : 2);
     if (T.blocks[slot]) return *T.blocks[slot];
+    if (T.dev_built) {
+        auto R = std::make_unique<RowBlockTable>();
+        row_blocks_dev(T, *R, rb);
+        T.blocks[slot] = std::move(R);
+        return *T.blocks[slot];
+    }
     const i64 N = T.N;
     const size_t L = T.L;
     HIPOTF_REQUIRE(T.hJ.size() == (size_t)N * L, "internal: same-spin host table missing");
@@ -310,12 +316,15 @@ DevHam *make_dev_ham(DevBasis *b, int n, const std::vector<double> &h, const std
     std::vector<float> e32(e64.begin(), e64.end());
     H->eri32.upload(e32);
     const HostBasis &hb = b->host;
-    build_same_spin(H->tA, hb.A, H->h, H->eri, n);
+    H->h64.upload(H->h);
+    // same-spin tables on the device where the rows fit its in-LDS sort, else on the host
+    if (!build_same_spin_dev(H->tA, *b->pA, H->h64, H->eri64, n)) build_same_spin(H->tA, hb.A, H->h, H->eri, n);
     H->ptA = &H->tA;
     if (b->pB == b->pA) {
         H->ptB = &H->tA;
     } else {
-        build_same_spin(H->tB, hb.B, H->h, H->eri, n);
+        if (!build_same_spin_dev(H->tB, *b->pB, H->h64, H->eri64, n))
+            build_same_spin(H->tB, hb.B, H->h, H->eri, n);
         H->ptB = &H->tB;
     }
     // the default row blocks now, so that the first sigma does not pay for them
@@ -326,17 +335,7 @@ DevHam *make_dev_ham(DevBasis *b, int n, const std::vector<double> &h, const std
             row_blocks(*H->ptB, rb);
         }
     }
-    std::vector<double> jab((size_t)hb.A.N * n, 0.0);
-    for (i64 a = 0; a < hb.A.N; ++a) {
-        u64 s = hb.A.strs[a];
-        for (int r = 0; r < n; ++r) {
-            double v = 0.0;
-            for (int p = 0; p < n; ++p)
-                if ((s >> p) & 1) v += H->eri[((size_t)(p * n + p) * n + r) * n + r];
-            jab[(size_t)a * n + r] = v;
-        }
-    }
-    H->jab.upload(jab);
+    build_jab_dev(H->jab, *b->pA, n, H->eri64);
     return H.release();
 }
 

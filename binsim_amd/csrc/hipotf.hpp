@@ -175,13 +175,20 @@ struct SameSpinTable {
                               // g (sources >= g * N / SS_RANGES); P[.][SS_RANGES] = L
     DevBuf c64, c32;          // double / float [N][L]
     DevBuf gdiag;             // double[N]  same-spin diagonal G[I,I]
-    std::vector<int32_t> hJ;  // host copies of J and the FP64 coefficients (row blocks are
-    std::vector<double> hC;   // built from them on first use)
+    std::vector<int32_t> hJ;  // host copies of J and the FP64 coefficients when the table was
+    std::vector<double> hC;   // built on the host (row blocks are then built from them)
+    bool dev_built = false;   // built on the device (tables.hip): row blocks too
     std::unique_ptr<RowBlockTable> blocks[3];  // rb = 4, 8, 16
 };
 
 // row-block table of T for rb in {4, 8, 16}, built (and uploaded) on first use
 const RowBlockTable &row_blocks(SameSpinTable &T, int rb);
+// device construction (tables.hip): false when a row is too long for its in-LDS sort (then the
+// host build applies)
+struct DevSpin;
+bool build_same_spin_dev(SameSpinTable &T, const DevSpin &S, const DevBuf &h64, const DevBuf &eri64, int n);
+void row_blocks_dev(SameSpinTable &T, RowBlockTable &R, int rb);
+void build_jab_dev(DevBuf &jab, const DevSpin &A, int n, const DevBuf &eri64);
 
 constexpr int ERI_PAD = 256;  // zero entries appended to the device integral tables
 
@@ -190,6 +197,7 @@ struct DevHam {
     int n = 0;
     double ecore = 0.0;
     std::vector<double> h, eri;   // host copies: h[p*n+q], eri[((p*n+q)*n+r)*n+s] = (pq|rs)
+    DevBuf h64;                   // h on device
     DevBuf eri64, eri32;          // (pq|rs) on device
     DevBuf jab;                   // double[Na][n]: sum_{p in a} (pp|rr)
     DevBuf dw64, dw32;            // [Na][256]: sum_{p in a} (pp|rt) at r*n + t (n*n <= 256),
