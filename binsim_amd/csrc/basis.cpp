@@ -133,7 +133,11 @@ static void upload_spin(DevSpin &D, const SpinSpace &S) {
     D.nsx = S.nsx;
     D.ndx = S.ndx;
     D.strs.upload(S.strs);
-    if (S.nsx > 0 && S.N > 0) D.sx.upload(build_sx(S));
+    // single-excitation table built on the device (tables.hip), on the host past 32 orbitals
+    if (S.nsx > 0 && S.N > 0) {
+        if (S.n <= 32) build_sx_dev(D);
+        else D.sx.upload(build_sx(S));
+    }
 }
 
 DevBasis *make_dev_basis(const HostBasis &hb) {

@@ -189,6 +189,7 @@ struct DevSpin;
 bool build_same_spin_dev(SameSpinTable &T, const DevSpin &S, const DevBuf &h64, const DevBuf &eri64, int n);
 void row_blocks_dev(SameSpinTable &T, RowBlockTable &R, int rb);
 void build_jab_dev(DevBuf &jab, const DevSpin &A, int n, const DevBuf &eri64);
+void build_sx_dev(DevSpin &D);  // single-excitation table of one spin (n <= 32)
 
 constexpr int ERI_PAD = 256;  // zero entries appended to the device integral tables
 

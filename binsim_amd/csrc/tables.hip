@@ -227,6 +227,46 @@ __global__ __launch_bounds__(1024) void k_rb_union(const int32_t *__restrict__ J
             (int32_t)(o + (tid == SS_RANGES ? U : tb_lower(key, U, (i64)tid * N / SS_RANGES)));
 }
 
+// single-excitation table of one spin: sx[slot][I], slot = io (n - k) + iv, r = io-th occupied and
+// t = iv-th empty orbital of string I, packed with the colex rank of I - r + t and the parity of
+// the occupied orbitals strictly between r and t
+__global__ __launch_bounds__(256) void k_sx(const u64 *__restrict__ strs, i64 N, int n, int k,
+                                            u32 *__restrict__ sx) {
+    __shared__ u32 bn[TB_BW * TB_BW];
+    for (int t = threadIdx.x; t < TB_BW * TB_BW; t += blockDim.x) bn[t] = tb_binom(t / TB_BW, t % TB_BW);
+    __syncthreads();
+    const int nv = n - k;
+    const i64 tot = N * (i64)(k * nv);
+    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < tot; x += (i64)gridDim.x * blockDim.x) {
+        const i64 slot = x / N, I = x - slot * N;
+        const int io = (int)(slot / nv), iv = (int)(slot - (i64)io * nv);
+        const u64 s = strs[I];
+        u64 m = s, e = ~s & ((n < 64) ? ((1ull << n) - 1) : ~0ull);
+        for (int i = 0; i < io; ++i) m &= m - 1;
+        for (int i = 0; i < iv; ++i) e &= e - 1;
+        const int r = __builtin_ctzll(m), t = __builtin_ctzll(e);
+        const u64 j = s ^ (1ull << r) ^ (1ull << t);
+        u32 rank = 0;
+        int q = 0;
+        for (u64 y = j; y; y &= y - 1, ++q) rank += bn[__builtin_ctzll(y) * TB_BW + q + 1];
+        const int lo = r < t ? r : t, hi = r < t ? t : r;
+        const bool neg = __popcll(s & ((1ull << hi) - 1) & ~((1ull << (lo + 1)) - 1)) & 1;
+        sx[x] = sx_pack(rank, r, t, neg);
+    }
+}
+
+void build_sx_dev(DevSpin &D) {
+    const i64 tot = D.N * (i64)D.nsx;
+    HIPOTF_REQUIRE(D.N < (1 << 20), "too many strings per spin for the packed tables (2^20)");
+    HIPOTF_REQUIRE(D.n <= 32, "internal: excitation table build for n > 32");
+    D.sx.alloc((size_t)tot * sizeof(u32));
+    if (tot == 0) return;
+    const i64 nb = std::min<i64>((tot + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_sx, dim3((unsigned)nb), dim3(256), 0, 0, D.strs.as<u64>(), D.N, D.n, D.k, D.sx.as<u32>());
+    HIPOTF_CHECK(hipGetLastError());
+    HIPOTF_CHECK(hipDeviceSynchronize());
+}
+
 __global__ __launch_bounds__(256) void k_jab(const u64 *__restrict__ strs, i64 N, int n,
                                              const double *__restrict__ eri, double *__restrict__ jab) {
     const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
