@@ -46,12 +46,12 @@ DEFAULT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.
 
 _RANGES = {"ab_kernel": (0, 1, 2, 3, 4, 5), "ab_split": (1, 2), "ab_db": (1, 2), "ab_nvs": (1, 2),
            "rs_passes": (0, 1, 2, 4, 8), "deterministic": (0, 1), "rs_block": (0, 4, 8, 16),
-           "sweep_mode": (0, 1, 2, 3)}
+           "sweep_mode": tuple(range(8))}
 
 
 def validate(cfg: dict) -> dict:
     """Return ``cfg`` completed with defaults; raise ValueError on an unknown or bad field."""
-    out = dict(zip(FIELDS, (0, 1, 1, 2, 0, 0, 8, 0)))
+    out = dict(zip(FIELDS, (0, 1, 1, 2, 0, 0, 8, 4)))
     for k, v in cfg.items():
         if k not in _RANGES:
             raise ValueError(f"unknown tuning field {k!r}")

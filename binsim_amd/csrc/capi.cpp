@@ -652,7 +652,7 @@ int hipotf_set_tuning(int precision, const int32_t *v, int n) {
                    "tuning: rs_passes must be 0, 1, 2, 4 or 8");
     HIPOTF_REQUIRE(v[5] == 0 || v[5] == 1, "tuning: deterministic must be 0 or 1");
     HIPOTF_REQUIRE(v[6] == 0 || v[6] == 4 || v[6] == 8 || v[6] == 16, "tuning: rs_block must be 0, 4, 8 or 16");
-    HIPOTF_REQUIRE(v[7] >= 0 && v[7] <= 3, "tuning: sweep_mode must be 0..3");
+    HIPOTF_REQUIRE(v[7] >= 0 && v[7] <= 7, "tuning: sweep_mode must be 0..7");
     SigmaTuning &t = tuning(precision);
     t.ab_kernel = v[0];
     t.ab_split = v[1];

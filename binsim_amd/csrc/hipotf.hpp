@@ -232,11 +232,14 @@ struct SigmaTuning {
     int deterministic = 0;  // 1: alpha-beta scatter wave by wave (bitwise reproducible)
     int rs_block = 8;       // row SpMM: destination rows sharing each gathered source row
                             // (4, 8, 16), or 0 for one row per wave
-    int sweep_mode = 0;     // VQE rotation sweeps (FP64 table): 0 fused segments with as many
+    int sweep_mode = 4;     // VQE rotation sweeps (FP64 table): 0 fused segments with as many
                             // rows (and the leading part of one more) in LDS as fit, the
                             // others in global memory; 1 fused, rows
                             // always in global memory; 2 one launch per generator; 3 fused,
-                            // at most one row and half of the next per block in LDS
+                            // at most one row and half of the next per block in LDS; +4 (4..7):
+                            // the same with the adjoint sweep's pair segments on interleaved
+                            // (l, r) rows (the default: half the scattered accesses where rows
+                            // miss the LDS)
 };
 SigmaTuning &tuning(int precision);  // 0: FP64 kernels, 1: FP32 kernels
 // timing of the sigma builds' alpha-beta sections (hipotf_ab_timing): while on, HIP events on
@@ -289,6 +292,7 @@ struct DevPool {
     DevBuf scal;                   // a few device doubles
     DevBuf segdesc, segpart;       // fused sweeps: per-position descriptors, per-block partials
     DevBuf tl, tr;                 // fused sweeps: beta-major copies of the two state vectors
+    DevBuf tz;                     // fused adjoint sweep on interleaved rows: (l, r) per element
     DevBuf vqe_res;                // fused VQE objective: per-parameter results + norms (grown, kept)
     DevBuf zeros;                  // batch tran: a zero angle per generator
 };

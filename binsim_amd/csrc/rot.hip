@@ -641,10 +641,105 @@ __global__ __launch_bounds__(256) void k_seg_reduce(const double *__restrict__ p
     if (threadIdx.x == 0) dres[k] = t;
 }
 
-// out[c * R + r] = in[r * Cc + c]  (R x Cc -> Cc x R)
-__global__ __launch_bounds__(256) void k_transpose(const double *__restrict__ in, double *__restrict__ out,
-                                                   i64 R, i64 Cc) {
-    __shared__ double tile[32][33];
+// ---- adjoint pair segments on interleaved rows: element e of the state holds (l[e], r[e]) as
+// one 16-byte value, so each side of a rotation pair is one access instead of two (the accesses
+// that miss the LDS are scattered, and each moves a whole sector whatever its width).  Row
+// segments keep separate l and r rows: a whole l row in LDS serves them better.
+template <int M>
+struct SegRow2 {
+    double2 *s, *g;
+    i64 sp;
+    __device__ __forceinline__ double2 ld(i64 i) const {
+        if constexpr (M == 0) return s[i];
+        else if constexpr (M == 2) return g[i];
+        else return i < sp ? s[i] : g[i];
+    }
+    __device__ __forceinline__ void st(i64 i, double2 v) const {
+        if constexpr (M == 0) s[i] = v;
+        else if constexpr (M == 2) g[i] = v;
+        else if (i < sp) s[i] = v;
+        else g[i] = v;
+    }
+};
+
+// one adjoint step of a pair: a = (l_u, r_u), b = (l_w, r_w); the arithmetic of k_seg_pair
+__device__ __forceinline__ void adj_rot(double c, double s, bool neg, double2 &a, double2 &b, double &acc) {
+    const double ts = neg ? -s : s, tc = neg ? -c : c;
+    const double nlu = c * a.x + ts * b.x, nlw = -ts * a.x + c * b.x;
+    acc += nlu * (-s * a.y + tc * b.y) + nlw * (-s * b.y - tc * a.y);
+    const double nru = c * a.y + ts * b.y, nrw = -ts * a.y + c * b.y;
+    a = make_double2(nlu, nru);
+    b = make_double2(nlw, nrw);
+}
+
+// k_seg_pair<true, *> on interleaved rows: rows u, w of the alpha pair; NST (0..2) of them
+// staged whole, the next one up to sp elements
+template <int NST>
+__global__ __launch_bounds__(1024) void k_seg_pair2(double2 *__restrict__ z, const int32_t *__restrict__ alist,
+                                                    i64 npairs, i64 NB, const u64 *__restrict__ strsA,
+                                                    const u64 *__restrict__ strsB,
+                                                    const SegGen *__restrict__ gens, int ng,
+                                                    double *__restrict__ partial, i64 sp) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const i64 p = blockIdx.x;
+    if (p >= npairs) return;
+    const int tid = threadIdx.x, bd = blockDim.x;
+    const int32_t ua = alist[2 * p], wa = alist[2 * p + 1];
+    const u64 as = strsA[ua];
+    double2 *const g0 = z + (i64)ua * NB, *const g1 = z + (i64)wa * NB;
+    double2 *const sm = reinterpret_cast<double2 *>(smem_raw);
+    const SegRow2<seg_mode(0, NST)> z0{sm, g0, NST == 0 ? sp : NB};
+    const SegRow2<seg_mode(1, NST)> z1{sm + NB, g1, sp};
+    auto stage = [&](bool in) {
+        for (int k = 0; k < 2; ++k) {
+            const i64 m = k < NST ? NB : (k == NST ? sp : 0);
+            double2 *g = k == 0 ? g0 : g1, *sk = sm + (i64)k * NB;
+            for (i64 t = tid; t < m; t += bd) {
+                if (in) sk[t] = g[t];
+                else g[t] = sk[t];
+            }
+        }
+    };
+    stage(true);
+    __syncthreads();
+    for (int kk = 0; kk < ng; ++kk) {
+        const int k = ng - 1 - kk;
+        const SegGen sg = gens[k];
+        double acc = 0.0;
+        for (i64 j = tid; j < sg.len; j += bd) {
+            const int32_t ub = sg.lst[2 * j], wb = sg.lst[2 * j + 1];
+            const bool neg = jw_sign(sg.g, as, strsB[ub]);
+            double2 a = z0.ld(ub), b = z1.ld(wb);
+            adj_rot(sg.c, sg.s, neg, a, b, acc);
+            z0.st(ub, a);
+            z1.st(wb, b);
+        }
+        const double t = block_sum(acc);
+        if (tid == 0) partial[(i64)k * npairs + p] = t;
+        __syncthreads();
+    }
+    stage(false);
+}
+
+// z[i] = (l[i], r[i]) and back
+__global__ __launch_bounds__(256) void k_zip(const double *__restrict__ l, const double *__restrict__ r,
+                                             double2 *__restrict__ z, i64 n) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
+        z[i] = make_double2(l[i], r[i]);
+}
+__global__ __launch_bounds__(256) void k_unzip(const double2 *__restrict__ z, double *__restrict__ l,
+                                               double *__restrict__ r, i64 n) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
+        const double2 v = z[i];
+        l[i] = v.x;
+        r[i] = v.y;
+    }
+}
+
+// out[c * R + r] = in[r * Cc + c]  (R x Cc -> Cc x R), elements of type E
+template <typename E>
+__global__ __launch_bounds__(256) void k_transpose(const E *__restrict__ in, E *__restrict__ out, i64 R, i64 Cc) {
+    __shared__ E tile[32][33];
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     const i64 c0 = (i64)blockIdx.x * 32, r0 = (i64)blockIdx.y * 32;
     for (int i = 0; i < 4; ++i) {
@@ -673,19 +768,21 @@ static size_t seg_lds_budget() {
 }
 
 // group the sweep (generator indices in order) into fusable segments; nvec rows per element
-// (1 for the forward expm sweep, 2 for the adjoint sweep) decide what fits LDS
-static std::vector<Segment> plan_sweep(DevPool *P, const i64 *idxs, i64 n, int nvec) {
+// (1 for the forward expm sweep, 2 for the adjoint sweep) decide what fits LDS.  il: the
+// adjoint sweep's pair segments on interleaved (l, r) rows (two rows of 16-byte elements)
+static std::vector<Segment> plan_sweep(DevPool *P, const i64 *idxs, i64 n, int nvec, bool il = false) {
     const DevBasis *B = P->basis;
     const i64 NA = B->pA->N, NB = B->pB->N;
     const size_t budget = seg_lds_budget();
     // as many of a block's rows as fit the LDS are staged there, the others updated in place
     // in global memory; sweep_mode 1 stages none, 2 launches per generator, 3 stages one row
-    // and half of the next
-    const int mode = tuning(0).sweep_mode;
+    // and half of the next (+4: the adjoint sweep on interleaved rows)
+    const int mode = tuning(0).sweep_mode & 3;
+    int esz = 8;
     // whole rows, then the part of the next row that fits (at least 1024 elements); mode 3 at
     // most one whole row and half of the next
     auto stage = [&](Segment &s, i64 len) {
-        const i64 cap = mode == 1 ? 0 : (i64)(budget / 8);
+        const i64 cap = mode == 1 ? 0 : (i64)(budget / esz);
         s.nst = (int)std::min<i64>(s.nrow, cap / len);
         s.sp = 0;
         if (mode == 3) {
@@ -709,8 +806,10 @@ static std::vector<Segment> plan_sweep(DevPool *P, const i64 *idxs, i64 n, int n
             while (i + s.count < n && gen(i + s.count).kind == 2) ++s.count;
         } else if (G.kind == 3 || (G.kind == 1 && i + 1 < n && gen(i + 1).kind == 3 && gen(i + 1).pa == G.pa)) {
             s.kind = SEG_PAIR;
-            s.nrow = 2 * nvec;
+            s.nrow = il ? 2 : 2 * nvec;
+            esz = il ? 16 : 8;
             stage(s, NB);
+            esz = 8;
             while (i + s.count < n && gen(i + s.count).pa == G.pa && gen(i + s.count).kind != 2) ++s.count;
         } else if (G.kind == 1) {
             s.kind = SEG_ROW_B;
@@ -803,7 +902,7 @@ static void sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, con
             }
     };
     bool transposed = false;  // grad: the beta-major copies are made once (the vectors do not change)
-    auto run = [&](const std::vector<Segment> &segs, bool back) {
+    auto run = [&](const std::vector<Segment> &segs, bool back, bool il) {
         fill(segs);
         P->segdesc.ensure((size_t)std::max<i64>(n, 1) * sizeof(SegGen));
         if (n) HIPOTF_CHECK(hipMemcpyAsync(P->segdesc.p, host.data(), n * sizeof(SegGen), hipMemcpyHostToDevice, st));
@@ -814,12 +913,28 @@ static void sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, con
         }
         const SegGen *dg = P->segdesc.as<SegGen>();
         const int nvec = back ? 2 : 1;
+        // interleaved pair segments: z = (psi, lam) element by element, made before the first
+        // pair segment of a run of them and split back before a row segment or the end
+        double2 *z = nullptr;
+        bool zipped = false;
+        if (il) {
+            P->tz.ensure((size_t)dim * sizeof(double2));
+            z = P->tz.as<double2>();
+        }
+        auto zip = [&](bool on) {
+            if (on == zipped) return;
+            if (on) hipLaunchKernelGGL(k_zip, dim3(grid_for(dim)), dim3(256), 0, st, psi, lam, z, dim);
+            else hipLaunchKernelGGL(k_unzip, dim3(grid_for(dim)), dim3(256), 0, st, z, psi, lam, dim);
+            HIPOTF_CHECK(hipGetLastError());
+            zipped = on;
+        };
         // segments with rows in global memory: one block per CU (LDS reserved for that where no
         // row is staged), so the rows the blocks in flight work on (256 CUs x up to 4 rows)
         // stay within the L2 / Infinity Cache
         const size_t glob_smem = seg_lds_budget() / 2 + 4096;
         for (size_t si = 0; si < segs.size(); ++si) {
             const Segment &s = back ? segs[segs.size() - 1 - si] : segs[si];
+            zip(il && s.kind == SEG_PAIR);
             if (s.kind == SEG_SINGLE) {
                 for (int kk = 0; kk < s.count; ++kk) {
                     const int k = back ? s.count - 1 - kk : kk;
@@ -839,8 +954,9 @@ static void sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, con
                     P->tr.ensure(dim * sizeof(double));
                     dim3 g((unsigned)((NB + 31) / 32), (unsigned)((NA + 31) / 32));
                     if (!transposed) {
-                        hipLaunchKernelGGL(k_transpose, g, dim3(256), 0, st, psi, P->tl.as<double>(), NA, NB);
-                        if (back) hipLaunchKernelGGL(k_transpose, g, dim3(256), 0, st, lam, P->tr.as<double>(), NA, NB);
+                        hipLaunchKernelGGL(k_transpose<double>, g, dim3(256), 0, st, psi, P->tl.as<double>(), NA, NB);
+                        if (back)
+                            hipLaunchKernelGGL(k_transpose<double>, g, dim3(256), 0, st, lam, P->tr.as<double>(), NA, NB);
                         transposed = grad;
                     }
                     l = P->tl.as<double>();
@@ -879,8 +995,9 @@ static void sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, con
                 HIPOTF_CHECK(hipGetLastError());
                 if (!A && !grad) {
                     dim3 g((unsigned)((NA + 31) / 32), (unsigned)((NB + 31) / 32));
-                    hipLaunchKernelGGL(k_transpose, g, dim3(256), 0, st, P->tl.as<double>(), psi, NB, NA);
-                    if (back) hipLaunchKernelGGL(k_transpose, g, dim3(256), 0, st, P->tr.as<double>(), lam, NB, NA);
+                    hipLaunchKernelGGL(k_transpose<double>, g, dim3(256), 0, st, P->tl.as<double>(), psi, NB, NA);
+                    if (back)
+                        hipLaunchKernelGGL(k_transpose<double>, g, dim3(256), 0, st, P->tr.as<double>(), lam, NB, NA);
                 }
                 if (back)
                     hipLaunchKernelGGL(k_seg_reduce, dim3(s.count), dim3(256), 0, st, part, nrows, dres + s.first);
@@ -892,13 +1009,29 @@ static void sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, con
                     if (back) HIPOTF_CHECK(hipMemsetAsync(dres + s.first, 0, s.count * sizeof(double), st));
                     continue;
                 }
-                const size_t staged = (size_t)(NB * s.nst + s.sp) * sizeof(double);
+                const size_t staged = (size_t)(NB * s.nst + s.sp) * (il ? sizeof(double2) : sizeof(double));
                 const size_t smem = staged ? staged : glob_smem;
                 const int bd = smem > glob_smem / 2 ? 1024 : 256;
                 double *part = nullptr;
                 if (back) {
                     P->segpart.ensure((size_t)s.count * np * sizeof(double));
                     part = P->segpart.as<double>();
+                }
+                if (il) {
+                    auto go = [&](auto kern) {
+                        HIPOTF_CHECK(hipFuncSetAttribute((const void *)kern,
+                                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+                        hipLaunchKernelGGL(kern, dim3((unsigned)np), dim3(bd), smem, st, z, LA.uw.as<int32_t>(), np,
+                                           NB, B->pA->strs.as<u64>(), B->pB->strs.as<u64>(), dg + s.first,
+                                           s.count, part, s.sp);
+                    };
+                    if (s.nst == 0) go(k_seg_pair2<0>);
+                    else if (s.nst == 1) go(k_seg_pair2<1>);
+                    else go(k_seg_pair2<2>);
+                    HIPOTF_CHECK(hipGetLastError());
+                    hipLaunchKernelGGL(k_seg_reduce, dim3(s.count), dim3(256), 0, st, part, np, dres + s.first);
+                    HIPOTF_CHECK(hipGetLastError());
+                    continue;
                 }
                 auto go_pair = [&](auto kern, double *rr, double *pp) {
                     HIPOTF_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -930,13 +1063,18 @@ static void sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, con
                 HIPOTF_CHECK(hipGetLastError());
             }
         }
+        zip(false);
     };
     if (grad) {
-        run(plan_sweep(P, idxs, n, 2), true);
+        run(plan_sweep(P, idxs, n, 2), true, false);
         return;
     }
-    if (forward) run(plan_sweep(P, idxs, n, 1), false);
-    if (backward) run(plan_sweep(P, idxs, n, 2), true);
+    if (forward) run(plan_sweep(P, idxs, n, 1), false, false);
+    if (backward) {
+        // sweep_mode + 4: the pair segments on interleaved rows
+        const bool il = (tuning(0).sweep_mode & 4) != 0;
+        run(plan_sweep(P, idxs, n, 2, il), true, il);
+    }
 }
 
 void fused_sweeps(DevPool *P, i64 n, const i64 *idxs, const double *thetas, double *psi, double *lam,

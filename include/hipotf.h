@@ -132,8 +132,10 @@ int hipotf_prepare(void *basis, void *net);
  * within an XCD's L2), deterministic 0|1, same-spin destination rows per
  * gathered source row 4|8|16 (0: one row per wave), VQE rotation sweeps 0 fused (as many rows
  * in LDS as fit, the others in global memory) | 1 fused with rows in global memory | 2 one
- * launch per generator | 3 fused with at most one row and half of the next in LDS (FP64 table
- * only)}, for the FP64
+ * launch per generator | 3 fused with at most one row and half of the next in LDS, +4 (4..7)
+ * the same with the adjoint sweep's alpha-pair segments on interleaved (l, r) rows (default 4;
+ * FP64 table only)},
+ * for the FP64
  * (precision 0) or the
  * FP32 (precision 1) kernels.  binsim_amd/autotune.py measures the candidates and persists its
  * choice as JSON.  Return 0 on success. */

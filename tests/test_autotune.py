@@ -26,9 +26,10 @@ def test_defaults_and_per_precision_state(restore_tuning):
     d = AT.get_config(AT.FP64)
     assert set(d) == set(AT.FIELDS)
     assert d["ab_kernel"] == 0 and d["ab_split"] == 1 and d["rs_passes"] == 0 and d["rs_block"] == 8
+    assert d["sweep_mode"] == 4
     AT.set_config(AT.FP32, dict(ab_kernel=1, ab_split=2, ab_db=2, ab_nvs=1, rs_passes=4, rs_block=16))
     assert AT.get_config(AT.FP32) == dict(ab_kernel=1, ab_split=2, ab_db=2, ab_nvs=1, rs_passes=4,
-                                          deterministic=0, rs_block=16, sweep_mode=0)
+                                          deterministic=0, rs_block=16, sweep_mode=4)
     assert AT.get_config(AT.FP64) == d  # the FP64 table is untouched
 
 
@@ -37,7 +38,7 @@ def test_invalid_values_rejected_by_library_and_host(restore_tuning):
     before = AT.get_config(AT.FP64)
     for bad in ([6, 1, 1, 2, 1, 0, 8, 0], [0, 3, 1, 2, 1, 0, 8, 0], [0, 1, 0, 2, 1, 0, 8, 0],
                 [0, 1, 1, 3, 1, 0, 8, 0], [0, 1, 1, 2, 3, 0, 8, 0], [0, 1, 1, 2, 1, 2, 8, 0],
-                [0, 1, 1, 2, 1, 0, 2, 0], [0, 1, 1, 2, 1, 0, 8, 4]):
+                [0, 1, 1, 2, 1, 0, 2, 0], [0, 1, 1, 2, 1, 0, 8, 8]):
         v = np.array(bad, dtype=np.int32)
         assert lib.hipotf_set_tuning(0, v.ctypes.data_as(L._i32p), 8) != 0
         assert "tuning" in L.last_error()

@@ -75,14 +75,16 @@ def test_fused_vqe_energy_grad(torch_cuda, n, na, nb):
     assert np.max(np.abs(host(psi) - lv)) < 1e-12  # reverse sweep uncomputes the state
 
 
-@pytest.mark.parametrize("sweep_mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("sweep_mode", [0, 1, 2, 3, 4, 5, 7])
 def test_vqe_config2_matches_reference_fixture(torch_cuda, sweep_mode):
     """12 orbitals, 6+6 electrons, all 1,818 UCC parameters, theta ~ U[-0.05, 0.05]: the fused
     energy + adjoint gradient equal the values the compiled reference produced (golden), for
     each sweep form: fused segments with rows in LDS (0; everything fits at this size), fused
     segments updating the rows in global memory (1), one launch per generator (2), and
     segments with one row and half of the next per block in LDS and the rest in global memory
-    (3; the forms 16 orbitals takes: whole, split and global rows)."""
+    (3; the forms 16 orbitals takes: whole, split and global rows); 4, 5, 7 are 0, 1, 3 with
+    the adjoint sweep's alpha-pair segments on interleaved (l, r) rows."""
+    print(f"vqe fixture sweep_mode={sweep_mode}", flush=True)
     import os
     from binsim_amd import autotune as AT
     f = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config0_n12.npz"))
