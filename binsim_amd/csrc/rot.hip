@@ -465,6 +465,41 @@ struct SegGen {
 
 enum { SEG_ROW_A = 0, SEG_ROW_B = 1, SEG_PAIR = 2, SEG_SINGLE = 3 };
 
+// A thread's first pair of a generator, (u, w) and the element string of u, loaded while the
+// previous generator runs (list(): the pair, early in that generator; str(): the string, once the
+// pair has arrived) so that the two dependent global loads overlap its work and barrier
+struct SegPrefetch {
+    int32_t u = 0, w = 0;
+    u64 e = 0;
+    __device__ __forceinline__ void list(const SegGen &g, int tid) {
+        if (tid < g.len) {
+            u = g.lst[2 * tid];
+            w = g.lst[2 * tid + 1];
+        }
+    }
+    __device__ __forceinline__ void str(const SegGen &g, int tid, const u64 *__restrict__ strs) {
+        if (tid < g.len) e = strs[u];
+    }
+};
+
+// end of a generator in a segment block: the block barrier, and with SUM the block's partial of
+// acc (wave sums in red, then thread 0 adds them in wave order, after the barrier, while the
+// other waves go on; red alternates by generator, so the next generator's wave sums do not
+// overwrite it before the following barrier)
+template <bool SUM>
+__device__ __forceinline__ void seg_partial(double acc, double *red, double *__restrict__ partial, i64 at) {
+    if (SUM) {
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    }
+    __syncthreads();  // (also orders this generator's global updates before the next one's reads)
+    if (SUM && threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += red[i];
+        partial[at] = t;
+    }
+}
+
 // A row of a segment block: in LDS (M = 0), in global memory (M = 2), or split (M = 1): its
 // first sp elements staged in LDS, the rest in global memory
 template <int M>
@@ -521,13 +556,19 @@ __global__ __launch_bounds__(1024) void k_seg_row(double *__restrict__ l, double
     };
     stage(true);
     __syncthreads();
+    SegPrefetch pf;
+    if (ng > 0) {
+        pf.list(gens[BACK ? ng - 1 : 0], tid);
+        pf.str(gens[BACK ? ng - 1 : 0], tid, el_strs);
+    }
+    __shared__ double red[2][16];
     for (int kk = 0; kk < ng; ++kk) {
         const int k = BACK ? ng - 1 - kk : kk;
         const SegGen sg = gens[k];
+        const SegPrefetch cur = pf;
+        if (kk + 1 < ng) pf.list(gens[BACK ? k - 1 : k + 1], tid);
         double acc = 0.0;
-        for (i64 j = tid; j < sg.len; j += bd) {
-            const int32_t u = sg.lst[2 * j], w = sg.lst[2 * j + 1];
-            const u64 es = el_strs[u];
+        auto body = [&](int32_t u, int32_t w, u64 es) {
             const bool neg = row_is_alpha ? jw_sign(sg.g, rs, es) : jw_sign(sg.g, es, rs);
             const double ts = neg ? -sg.s : sg.s;
             if (GRAD) {
@@ -549,12 +590,14 @@ __global__ __launch_bounds__(1024) void k_seg_row(double *__restrict__ l, double
                 xr.st(u, sg.c * ru + ts * rw);
                 xr.st(w, -ts * ru + sg.c * rw);
             }
+        };
+        if (tid < sg.len) body(cur.u, cur.w, cur.e);
+        for (i64 j = tid + bd; j < sg.len; j += bd) {
+            const int32_t u = sg.lst[2 * j], w = sg.lst[2 * j + 1];
+            body(u, w, el_strs[u]);
         }
-        if (BACK) {
-            const double t = block_sum(acc);
-            if (tid == 0) partial[(i64)k * nrows + row] = t;
-        }
-        __syncthreads();  // (also orders this generator's global updates before the next one's reads)
+        if (kk + 1 < ng) pf.str(gens[BACK ? k - 1 : k + 1], tid, el_strs);
+        seg_partial<BACK>(acc, red[kk & 1], partial, (i64)k * nrows + row);
     }
     if (!GRAD) stage(false);
 }
@@ -595,13 +638,20 @@ __global__ __launch_bounds__(1024) void k_seg_pair(double *__restrict__ l, doubl
     };
     stage(true);
     __syncthreads();
+    SegPrefetch pf;
+    if (ng > 0) {
+        pf.list(gens[BACK ? ng - 1 : 0], tid);
+        pf.str(gens[BACK ? ng - 1 : 0], tid, strsB);
+    }
+    __shared__ double red[2][16];
     for (int kk = 0; kk < ng; ++kk) {
         const int k = BACK ? ng - 1 - kk : kk;
         const SegGen sg = gens[k];
+        const SegPrefetch cur = pf;
+        if (kk + 1 < ng) pf.list(gens[BACK ? k - 1 : k + 1], tid);
         double acc = 0.0;
-        for (i64 j = tid; j < sg.len; j += bd) {
-            const int32_t ub = sg.lst[2 * j], wb = sg.lst[2 * j + 1];
-            const bool neg = jw_sign(sg.g, as, strsB[ub]);
+        auto body = [&](int32_t ub, int32_t wb, u64 bs) {
+            const bool neg = jw_sign(sg.g, as, bs);
             const double ts = neg ? -sg.s : sg.s;
             if (GRAD) {
                 const double tc = neg ? -sg.c : sg.c;
@@ -621,12 +671,14 @@ __global__ __launch_bounds__(1024) void k_seg_pair(double *__restrict__ l, doubl
                 r0.st(ub, sg.c * ru + ts * rw);
                 r1.st(wb, -ts * ru + sg.c * rw);
             }
+        };
+        if (tid < sg.len) body(cur.u, cur.w, cur.e);
+        for (i64 j = tid + bd; j < sg.len; j += bd) {
+            const int32_t ub = sg.lst[2 * j], wb = sg.lst[2 * j + 1];
+            body(ub, wb, strsB[ub]);
         }
-        if (BACK) {
-            const double t = block_sum(acc);
-            if (tid == 0) partial[(i64)k * npairs + p] = t;
-        }
-        __syncthreads();  // (also orders this generator's global updates before the next one's reads)
+        if (kk + 1 < ng) pf.str(gens[BACK ? k - 1 : k + 1], tid, strsB);
+        seg_partial<BACK>(acc, red[kk & 1], partial, (i64)k * npairs + p);
     }
     if (!GRAD) stage(false);
 }
@@ -702,21 +754,31 @@ __global__ __launch_bounds__(1024) void k_seg_pair2(double2 *__restrict__ z, con
     };
     stage(true);
     __syncthreads();
+    SegPrefetch pf;
+    if (ng > 0) {
+        pf.list(gens[ng - 1], tid);
+        pf.str(gens[ng - 1], tid, strsB);
+    }
+    __shared__ double red[2][16];
     for (int kk = 0; kk < ng; ++kk) {
         const int k = ng - 1 - kk;
         const SegGen sg = gens[k];
+        const SegPrefetch cur = pf;
+        if (kk + 1 < ng) pf.list(gens[k - 1], tid);
         double acc = 0.0;
-        for (i64 j = tid; j < sg.len; j += bd) {
-            const int32_t ub = sg.lst[2 * j], wb = sg.lst[2 * j + 1];
-            const bool neg = jw_sign(sg.g, as, strsB[ub]);
+        auto body = [&](int32_t ub, int32_t wb, u64 bs) {
             double2 a = z0.ld(ub), b = z1.ld(wb);
-            adj_rot(sg.c, sg.s, neg, a, b, acc);
+            adj_rot(sg.c, sg.s, jw_sign(sg.g, as, bs), a, b, acc);
             z0.st(ub, a);
             z1.st(wb, b);
+        };
+        if (tid < sg.len) body(cur.u, cur.w, cur.e);
+        for (i64 j = tid + bd; j < sg.len; j += bd) {
+            const int32_t ub = sg.lst[2 * j], wb = sg.lst[2 * j + 1];
+            body(ub, wb, strsB[ub]);
         }
-        const double t = block_sum(acc);
-        if (tid == 0) partial[(i64)k * npairs + p] = t;
-        __syncthreads();
+        if (kk + 1 < ng) pf.str(gens[k - 1], tid, strsB);
+        seg_partial<true>(acc, red[kk & 1], partial, (i64)k * npairs + p);
     }
     stage(false);
 }
