@@ -467,7 +467,10 @@ enum { SEG_ROW_A = 0, SEG_ROW_B = 1, SEG_PAIR = 2, SEG_SINGLE = 3 };
 
 // A thread's first pair of a generator, (u, w) and the element string of u, loaded while the
 // previous generator runs (list(): the pair, early in that generator; str(): the string, once the
-// pair has arrived) so that the two dependent global loads overlap its work and barrier
+// pair has arrived) so that the two dependent global loads overlap its work and barrier.  Row
+// segments use it (a 16-orbital row generator is one pair per thread, so these loads are most of
+// its latency); pair segments do not: there the early loads delay the first pair's reads of the
+// rows in global memory more than they save
 struct SegPrefetch {
     int32_t u = 0, w = 0;
     u64 e = 0;
@@ -638,17 +641,10 @@ __global__ __launch_bounds__(1024) void k_seg_pair(double *__restrict__ l, doubl
     };
     stage(true);
     __syncthreads();
-    SegPrefetch pf;
-    if (ng > 0) {
-        pf.list(gens[BACK ? ng - 1 : 0], tid);
-        pf.str(gens[BACK ? ng - 1 : 0], tid, strsB);
-    }
     __shared__ double red[2][16];
     for (int kk = 0; kk < ng; ++kk) {
         const int k = BACK ? ng - 1 - kk : kk;
         const SegGen sg = gens[k];
-        const SegPrefetch cur = pf;
-        if (kk + 1 < ng) pf.list(gens[BACK ? k - 1 : k + 1], tid);
         double acc = 0.0;
         auto body = [&](int32_t ub, int32_t wb, u64 bs) {
             const bool neg = jw_sign(sg.g, as, bs);
@@ -672,12 +668,10 @@ __global__ __launch_bounds__(1024) void k_seg_pair(double *__restrict__ l, doubl
                 r1.st(wb, -ts * ru + sg.c * rw);
             }
         };
-        if (tid < sg.len) body(cur.u, cur.w, cur.e);
-        for (i64 j = tid + bd; j < sg.len; j += bd) {
+        for (i64 j = tid; j < sg.len; j += bd) {
             const int32_t ub = sg.lst[2 * j], wb = sg.lst[2 * j + 1];
             body(ub, wb, strsB[ub]);
         }
-        if (kk + 1 < ng) pf.str(gens[BACK ? k - 1 : k + 1], tid, strsB);
         seg_partial<BACK>(acc, red[kk & 1], partial, (i64)k * npairs + p);
     }
     if (!GRAD) stage(false);
@@ -754,17 +748,10 @@ __global__ __launch_bounds__(1024) void k_seg_pair2(double2 *__restrict__ z, con
     };
     stage(true);
     __syncthreads();
-    SegPrefetch pf;
-    if (ng > 0) {
-        pf.list(gens[ng - 1], tid);
-        pf.str(gens[ng - 1], tid, strsB);
-    }
     __shared__ double red[2][16];
     for (int kk = 0; kk < ng; ++kk) {
         const int k = ng - 1 - kk;
         const SegGen sg = gens[k];
-        const SegPrefetch cur = pf;
-        if (kk + 1 < ng) pf.list(gens[k - 1], tid);
         double acc = 0.0;
         auto body = [&](int32_t ub, int32_t wb, u64 bs) {
             double2 a = z0.ld(ub), b = z1.ld(wb);
@@ -772,12 +759,10 @@ __global__ __launch_bounds__(1024) void k_seg_pair2(double2 *__restrict__ z, con
             z0.st(ub, a);
             z1.st(wb, b);
         };
-        if (tid < sg.len) body(cur.u, cur.w, cur.e);
-        for (i64 j = tid + bd; j < sg.len; j += bd) {
+        for (i64 j = tid; j < sg.len; j += bd) {
             const int32_t ub = sg.lst[2 * j], wb = sg.lst[2 * j + 1];
             body(ub, wb, strsB[ub]);
         }
-        if (kk + 1 < ng) pf.str(gens[k - 1], tid, strsB);
         seg_partial<true>(acc, red[kk & 1], partial, (i64)k * npairs + p);
     }
     stage(false);
