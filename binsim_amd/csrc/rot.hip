@@ -485,6 +485,25 @@ struct SegPrefetch {
     }
 };
 
+// A generator's descriptor held one dword per lane (lanes 0..SEG_GEN_DW-1 of every wave), loaded
+// with a vector load some generators ahead and read back with readlane when its turn comes: a
+// scalar load issued that early would be waited for at the first LDS access after it (the two
+// share one wait counter), so it would not overlap anything
+constexpr int SEG_GEN_DW = (int)(sizeof(SegGen) / 4);
+static_assert(sizeof(SegGen) % 4 == 0 && SEG_GEN_DW <= 64, "descriptor must fit one wave, whole dwords");
+__device__ __forceinline__ uint32_t seg_gen_ld(const SegGen *__restrict__ gens, int k) {
+    const int lane = threadIdx.x & 63;
+    return lane < SEG_GEN_DW ? reinterpret_cast<const uint32_t *>(gens + k)[lane] : 0u;
+}
+__device__ __forceinline__ SegGen seg_gen_get(uint32_t d) {
+    uint32_t w[SEG_GEN_DW];
+#pragma unroll
+    for (int i = 0; i < SEG_GEN_DW; ++i) w[i] = (uint32_t)__builtin_amdgcn_readlane((int)d, i);
+    SegGen g;
+    __builtin_memcpy(&g, w, sizeof(SegGen));
+    return g;
+}
+
 // end of a generator in a segment block: the block barrier, and with SUM the block's partial of
 // acc (wave sums in red, then thread 0 adds them in wave order, after the barrier, while the
 // other waves go on; red alternates by generator, so the next generator's wave sums do not
@@ -559,17 +578,22 @@ __global__ __launch_bounds__(1024) void k_seg_row(double *__restrict__ l, double
     };
     stage(true);
     __syncthreads();
+    auto gen_at = [&](int kk) { return BACK ? ng - 1 - kk : kk; };
+    uint32_t dc = ng > 0 ? seg_gen_ld(gens, gen_at(0)) : 0u, dn = ng > 1 ? seg_gen_ld(gens, gen_at(1)) : 0u;
     SegPrefetch pf;
     if (ng > 0) {
-        pf.list(gens[BACK ? ng - 1 : 0], tid);
-        pf.str(gens[BACK ? ng - 1 : 0], tid, el_strs);
+        const SegGen g0 = seg_gen_get(dc);
+        pf.list(g0, tid);
+        pf.str(g0, tid, el_strs);
     }
     __shared__ double red[2][16];
     for (int kk = 0; kk < ng; ++kk) {
-        const int k = BACK ? ng - 1 - kk : kk;
-        const SegGen sg = gens[k];
+        const int k = gen_at(kk);
+        const SegGen sg = seg_gen_get(dc), sn = seg_gen_get(dn);  // sn: the next one (kk + 1 < ng)
+        dc = dn;
+        if (kk + 2 < ng) dn = seg_gen_ld(gens, gen_at(kk + 2));
         const SegPrefetch cur = pf;
-        if (kk + 1 < ng) pf.list(gens[BACK ? k - 1 : k + 1], tid);
+        if (kk + 1 < ng) pf.list(sn, tid);
         double acc = 0.0;
         auto body = [&](int32_t u, int32_t w, u64 es) {
             const bool neg = row_is_alpha ? jw_sign(sg.g, rs, es) : jw_sign(sg.g, es, rs);
@@ -599,7 +623,7 @@ __global__ __launch_bounds__(1024) void k_seg_row(double *__restrict__ l, double
             const int32_t u = sg.lst[2 * j], w = sg.lst[2 * j + 1];
             body(u, w, el_strs[u]);
         }
-        if (kk + 1 < ng) pf.str(gens[BACK ? k - 1 : k + 1], tid, el_strs);
+        if (kk + 1 < ng) pf.str(sn, tid, el_strs);
         seg_partial<BACK>(acc, red[kk & 1], partial, (i64)k * nrows + row);
     }
     if (!GRAD) stage(false);
@@ -642,9 +666,18 @@ __global__ __launch_bounds__(1024) void k_seg_pair(double *__restrict__ l, doubl
     stage(true);
     __syncthreads();
     __shared__ double red[2][16];
+    // descriptors loaded ahead in the adjoint and gradient forms; the forward form's first global
+    // row reads would queue behind that load, so it takes each descriptor when it starts
+    auto gen_at = [&](int kk) { return BACK ? ng - 1 - kk : kk; };
+    uint32_t dc = BACK && ng > 0 ? seg_gen_ld(gens, gen_at(0)) : 0u;
+    uint32_t dn = BACK && ng > 1 ? seg_gen_ld(gens, gen_at(1)) : 0u;
     for (int kk = 0; kk < ng; ++kk) {
-        const int k = BACK ? ng - 1 - kk : kk;
-        const SegGen sg = gens[k];
+        const int k = gen_at(kk);
+        const SegGen sg = BACK ? seg_gen_get(dc) : gens[k];
+        if (BACK) {
+            dc = dn;
+            if (kk + 2 < ng) dn = seg_gen_ld(gens, gen_at(kk + 2));
+        }
         double acc = 0.0;
         auto body = [&](int32_t ub, int32_t wb, u64 bs) {
             const bool neg = jw_sign(sg.g, as, bs);
@@ -749,9 +782,12 @@ __global__ __launch_bounds__(1024) void k_seg_pair2(double2 *__restrict__ z, con
     stage(true);
     __syncthreads();
     __shared__ double red[2][16];
+    uint32_t dc = ng > 0 ? seg_gen_ld(gens, ng - 1) : 0u, dn = ng > 1 ? seg_gen_ld(gens, ng - 2) : 0u;
     for (int kk = 0; kk < ng; ++kk) {
         const int k = ng - 1 - kk;
-        const SegGen sg = gens[k];
+        const SegGen sg = seg_gen_get(dc);
+        dc = dn;
+        if (kk + 2 < ng) dn = seg_gen_ld(gens, k - 2);
         double acc = 0.0;
         auto body = [&](int32_t ub, int32_t wb, u64 bs) {
             double2 a = z0.ld(ub), b = z1.ld(wb);
