@@ -460,6 +460,21 @@ using namespace hipotf;
 
 static const double kNaN = std::numeric_limits<double>::quiet_NaN();
 
+// index ia * Nb + ib of determinant |astr, bstr> in the basis, or -1 if it is not there
+// (basis.hpp:298-330: strings outside the space are ignored)
+static i64 det_index(const HostBasis *hb, uint32_t astr, uint32_t bstr) {
+    if (hb->dim == 0) return -1;
+    auto ia = std::lower_bound(hb->A.strs.begin(), hb->A.strs.end(), (u64)astr);
+    auto ib = std::lower_bound(hb->B.strs.begin(), hb->B.strs.end(), (u64)bstr);
+    if (ia == hb->A.strs.end() || *ia != astr || ib == hb->B.strs.end() || *ib != bstr) return -1;
+    return (i64)(ia - hb->A.strs.begin()) * hb->B.N + (i64)(ib - hb->B.strs.begin());
+}
+
+// complex float64 as passed by value across the boundary (two doubles: real, imaginary)
+struct HipotfC64 {
+    double re, im;
+};
+
 extern "C" {
 
 const char *hipotf_last_error(void) { return last_error_cstr(); }
@@ -492,12 +507,20 @@ void destroy_basis_manager(void *basis) { delete static_cast<HostBasis *>(basis)
 void set_det_coeff_f64(void *basis, uint32_t astr, uint32_t bstr, double coeff, double *vec) {
     API_TRY
     HIPOTF_REQUIRE(basis && vec, "NULL argument");
-    const HostBasis *hb = static_cast<HostBasis *>(basis);
-    if (hb->dim == 0) return;
-    auto ia = std::lower_bound(hb->A.strs.begin(), hb->A.strs.end(), (u64)astr);
-    auto ib = std::lower_bound(hb->B.strs.begin(), hb->B.strs.end(), (u64)bstr);
-    if (ia == hb->A.strs.end() || *ia != astr || ib == hb->B.strs.end() || *ib != bstr) return;
-    vec[(ia - hb->A.strs.begin()) * hb->B.N + (ib - hb->B.strs.begin())] += coeff;
+    const i64 at = det_index(static_cast<HostBasis *>(basis), astr, bstr);
+    if (at >= 0) vec[at] += coeff;
+    API_CATCH_VOID
+}
+
+// complex vector: (re, im) pairs, the layout of the reference's complexf64
+void set_det_coeff_c64(void *basis, uint32_t astr, uint32_t bstr, HipotfC64 coeff, HipotfC64 *vec) {
+    API_TRY
+    HIPOTF_REQUIRE(basis && vec, "NULL argument");
+    const i64 at = det_index(static_cast<HostBasis *>(basis), astr, bstr);
+    if (at >= 0) {
+        vec[at].re += coeff.re;
+        vec[at].im += coeff.im;
+    }
     API_CATCH_VOID
 }
 

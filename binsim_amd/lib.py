@@ -17,6 +17,11 @@ _u32p = C.POINTER(C.c_uint32)
 _dpp = C.POINTER(C.c_void_p)
 _vp = C.c_void_p
 
+
+class C64(C.Structure):
+    """complex float64 by value (real, imaginary), the reference's complexf64"""
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
 # name -> (restype, argtypes); every symbol include/hipotf.h declares
 SIGNATURES = {
     "hipotf_last_error": (C.c_char_p, []),
@@ -26,6 +31,7 @@ SIGNATURES = {
     "get_num_symmetry_blocks": (C.c_int64, [_vp]),
     "destroy_basis_manager": (None, [_vp]),
     "set_det_coeff_f64": (None, [_vp, C.c_uint32, C.c_uint32, C.c_double, _vp]),
+    "set_det_coeff_c64": (None, [_vp, C.c_uint32, C.c_uint32, C64, _vp]),
     "build_network_otf_f64": (_vp, [_i64p, C.c_int64, C.c_int64, _u32p, _u32p, _i64p, _i64p, _i64p,
                                     _u32p, _u32p, _dp, _dp]),
     "destroy_network_otf_f64": (None, [_vp]),

@@ -39,6 +39,13 @@ int64_t get_num_symmetry_blocks(void *basis);   /* basis/declare.hpp:11-14 */
 void destroy_basis_manager(void *basis);        /* basis/declare.hpp:16-21 */
 /* set_det_coeff_f64: basis/declare.hpp:35-45 -> basis.hpp:298-330 (vec is a HOST array) */
 void set_det_coeff_f64(void *basis, uint32_t astr, uint32_t bstr, double coeff, double *vec);
+/* complex float64, the layout of the reference's complexf64 (real, imaginary) */
+typedef struct {
+    double re, im;
+} HipotfC64;
+/* set_det_coeff_c64: basis/declare.hpp:46-56 -> basis.hpp:298-330 on a complex HOST vector
+ * (adds coeff at the determinant's index; determinants outside the basis are ignored) */
+void set_det_coeff_c64(void *basis, uint32_t astr, uint32_t bstr, HipotfC64 coeff, HipotfC64 *vec);
 
 /* ---- host operator network: replaces build/destroy_network_otf_f64 of libotf_native.so
  *      (src/include/op/declare.hpp:15-38, ham/otf.hpp:223-349).  Same flat SVD-group arrays

@@ -45,6 +45,12 @@ def test_host_basis_matches_reference_semantics():
     # strings outside the basis are ignored
     lib.set_det_coeff_f64(b.ptr, 0b000011, 0b000101, 1.0, v.ctypes.data)
     assert v.sum() == 3.0
+    # the complex form (basis/declare.hpp:46-56) on a complex128 array: same index, both parts
+    z = np.zeros(b.dim, dtype=np.complex128)
+    lib.set_det_coeff_c64(b.ptr, 0b110001, 0b000101, L.C64(1.5, -2.0), z.ctypes.data)
+    lib.set_det_coeff_c64(b.ptr, 0b110001, 0b000101, L.C64(0.5, 0.25), z.ctypes.data)
+    lib.set_det_coeff_c64(b.ptr, 0b000011, 0b000101, L.C64(9.0, 9.0), z.ctypes.data)
+    assert z[16 * 15 + 1] == 2.0 - 1.75j and np.count_nonzero(z) == 1
 
 
 def test_empty_and_error_paths():
