@@ -8,7 +8,9 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libhipotf.so")
+# HIPOTF_LIBRARY names another build of the same library (the host-sanitizer build of
+# binsim_amd/build.py --asan); by default the in-tree libhipotf.so
+LIB_PATH = os.environ.get("HIPOTF_LIBRARY") or os.path.join(HERE, "libhipotf.so")
 
 _dp = C.POINTER(C.c_double)
 _i64p = C.POINTER(C.c_int64)
